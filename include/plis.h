@@ -1,0 +1,157 @@
+/*
+ * plis.h -- C ABI of libplis_hip.so, the gfx950 RDME tau-leap + SSA step.
+ *
+ * This is the drop-in boundary for PLISCuda's simulation path:
+ *   PLISCuda::run_simulation(Topology t, State s, Reactions r, float* h_rrc,
+ *       float* h_drc, float endTime, int compartments_count,
+ *       int* subv_constants, float log_freq)
+ *   (PLISCuda/include/cuda/driver.cuh:32-33, PLISCuda/src/driver.cu:12-408)
+ * whose only caller is PLISCuda/src/main.cpp:53-63.  The C++ driver in
+ * pliscuda_amd/csrc/host keeps that signature and calls the entry points
+ * below; a caller in another language binds them directly (ctypes, cgo, JNI).
+ *
+ * Conventions: plain pointers and sizes only; every function returns 0 on
+ * success or a negative PLIS_ERR_* code and never exits the process (the
+ * reference calls exit() from gpuAssert, PLISCuda/include/cpp/cpp_utils.hpp:97-105);
+ * plis_last_error() gives the message.  Host arrays are borrowed for the
+ * duration of a call and copied; the library owns all device memory and frees
+ * it in plis_destroy() (the reference never frees device memory).  One
+ * context = one HIP stream; contexts are independent (no globals such as the
+ * reference's __constant__ SBC/SPC/RC/NC/EPSILON, driver.cu:4-8), a context is
+ * not thread-safe.
+ */
+#ifndef PLIS_H_
+#define PLIS_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PLIS_OK 0
+#define PLIS_ERR_ARG (-1)      /* invalid argument / model                 */
+#define PLIS_ERR_HIP (-2)      /* HIP runtime error                        */
+#define PLIS_ERR_LEAPBACK (-3) /* a step needed more than 63 leap-backs    */
+#define PLIS_ERR_NODEV (-4)    /* no gfx950 device                         */
+#define PLIS_ERR_STATE (-5)    /* call not valid in the context's state    */
+
+/* Reaction network shared by all realizations of a context.
+ * Replaces Reactions (PLISCuda/include/cpp/Reactions.hpp:12-223; layout
+ * reactants/products [s*R + r], spec/reactions.md:58-80) and the per-
+ * compartment constants h_rrc / h_drc (driver.cu:98-104, read by
+ * cpp_utils.hpp:29-45).  Orders 0..3 are supported (the reference handles
+ * 1..2, rates.cu:3-39).  Propensity a_j = k_j * prod_s C(x_s, r_sj)
+ * (spec/rate_matrix.md:46-58); hop propensity along edge e of species s from
+ * a subvolume of compartment c = d[c*S+s] * x_s * w_e (rates.cu:55 has w=1). */
+typedef struct plis_network {
+    int32_t n_species;
+    int32_t n_reactions;
+    int32_t n_compartments;
+    const int32_t *reactants;  /* [s*R + j], >= 0                       */
+    const int32_t *products;   /* [s*R + j], >= 0                       */
+    const double *k;           /* [c*R + j] reaction constants          */
+    const double *d;           /* [c*S + s] per-edge diffusion constants */
+} plis_network;
+
+/* One realization's topology.  Replaces Topology's 6-slot self-padded rows
+ * (PLISCuda/include/cpp/Topology.hpp:98-191, driver.cu:150-161): CSR with any
+ * degree.  Self loops are dropped, duplicate neighbours merge (weights add). */
+typedef struct plis_topology {
+    uint32_t n_subvolumes;
+    uint32_t n_edges;
+    const uint32_t *row_ptr;   /* N+1                                    */
+    const uint32_t *col;       /* n_edges neighbour ids < N              */
+    const float *weight;       /* n_edges > 0, or NULL for all 1         */
+    const int32_t *compartment;/* N ids in [0, C), or NULL for all 0.
+                                  Replaces subv_constants (driver.cu:138-140) */
+} plis_topology;
+
+typedef struct plis_params {
+    double epsilon;            /* Cao epsilon (reference: 0.05 fixed, driver.cu:37) */
+    int32_t n_critical;        /* critical threshold n_c (reference: 10, driver.cu:36) */
+    int32_t waves_per_block;   /* 0 = default/autotuned                  */
+    int32_t blocks_per_cu;     /* 0 = default/autotuned                  */
+    int32_t use_graph;         /* 1 = replay the step as a hipGraph      */
+    uint64_t seed;             /* Philox key (reference: clock64()*v, init.cu:8) */
+} plis_params;
+
+typedef struct plis_stats {
+    double t;                  /* simulated time                         */
+    double tau;                /* last step's tau                        */
+    uint64_t steps;            /* accepted global steps                  */
+    uint64_t leapbacks;        /* rejected attempts (negative population)*/
+    uint64_t firings;          /* reaction events + molecule hops        */
+    uint64_t subvolume_leaps;  /* subvolumes advanced by accepted steps  */
+    uint64_t crit_events;      /* critical (SSA) events fired            */
+    int32_t status;            /* 0 running/paused, 1 done, 2 no event possible
+                                  (the reference's "min(tau) = +Inf" stop,
+                                  driver.cu:269-272), <0 error           */
+    int32_t pad;
+} plis_stats;
+
+typedef struct plis_ctx plis_ctx;
+
+/* Create a context holding n_real realizations (allocation and uploads of
+ * driver.cu:55-213).  topos[i] / x0[i] are realization i's topology and
+ * initial counts (species-major [s*N + v], uint32; the reference's State is
+ * int32 [s*N+v], State.hpp:112-217).  ids[i] is the realization id that keys
+ * its random streams (NULL = 0..n_real-1); ids must be < 65536.  */
+int plis_create(const plis_network *net, int32_t n_real, const plis_topology *topos,
+                const uint32_t *const *x0, const uint32_t *ids, const plis_params *params,
+                plis_ctx **out);
+
+/* Advance every realization until t >= t_end (the last step is clamped to
+ * t_end), or until it has made step_limit accepted steps in total.  This is
+ * the step loop of driver.cu:259-376. */
+int plis_run(plis_ctx *ctx, double t_end, uint64_t step_limit);
+
+/* As plis_run, but also stop a realization after the first accepted step
+ * that reaches t_pause (no clamping at t_pause); used for logFreq snapshots
+ * (driver.cu:362-375) without perturbing the trajectory. */
+int plis_run_until(plis_ctx *ctx, double t_end, double t_pause, uint64_t step_limit);
+
+/* Set every realization's stopping rule (as plis_run_until) without
+ * running; plis_iterate / plis_profile then advance armed realizations. */
+int plis_arm(plis_ctx *ctx, double t_end, double t_pause, uint64_t step_limit);
+
+/* Run exactly n step iterations (one tau pass + one leap pass per
+ * iteration, whatever the realizations' clocks) -- the benchmark's unit. */
+int plis_iterate(plis_ctx *ctx, uint32_t n);
+
+/* Time n iterations with HIP events recorded on the context's stream around
+ * every kernel launch; out[0] = total ms, out[1] = tau-kernel ms summed,
+ * out[2] = leap-kernel ms summed. */
+int plis_profile(plis_ctx *ctx, uint32_t n, double *out);
+
+int plis_set_state(plis_ctx *ctx, int32_t r, const uint32_t *x);
+int plis_get_state(plis_ctx *ctx, int32_t r, uint32_t *x);
+int plis_get_stats(plis_ctx *ctx, int32_t r, plis_stats *st);
+int plis_sync(plis_ctx *ctx);
+
+/* Per-subvolume tau', a0c (critical propensity) and tau'' of the current
+ * state of every realization, concatenated in realization order (kernel-level
+ * parity; the reference's DEBUG dumps of driver.cu:223-239). */
+int plis_debug_tau(plis_ctx *ctx, double *tau_prime, double *a0c, double *tau2);
+
+/* Launch shape in use: out[0] waves per block, out[1] blocks, out[2] tiles. */
+int plis_launch_shape(plis_ctx *ctx, int32_t *out);
+
+/* Autotune waves-per-block x blocks-per-CU on this context's workload by
+ * timing `iters` real iterations per candidate (results do not depend on
+ * the launch shape, so the trajectory is unchanged). */
+int plis_autotune(plis_ctx *ctx, uint32_t iters);
+
+void plis_destroy(plis_ctx *ctx);
+
+/* Message of the last failing call on ctx (ctx may be NULL for plis_create
+ * failures; that message is thread-local). */
+const char *plis_last_error(const plis_ctx *ctx);
+
+/* Number of gfx950 devices visible (0 when none); name buffer optional. */
+int plis_device_count(char *name, int32_t name_len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

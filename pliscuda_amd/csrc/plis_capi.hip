@@ -1,0 +1,732 @@
+// plis_capi.hip -- host side of libplis_hip.so: the C ABI declared in
+// include/plis.h.  Builds the canonical device layout of a batch of
+// realizations, launches the two step kernels (optionally replayed as a
+// hipGraph), and keeps every control decision on the device: the host only
+// polls a done-counter once per batch of iterations (the reference makes at
+// least seven blocking host<->device transactions per step,
+// PLISCuda/src/driver.cu:259-376).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/plis.h"
+#include "plis_kernels.hip"
+
+using namespace plis;
+
+static thread_local std::string g_create_err;
+
+struct plis_ctx {
+    hipStream_t stream = nullptr;
+    int device = 0;
+    int cus = 0;
+    std::string err;
+    NetHeader nh{};
+    int variant = 0;  // 0: <1,4>, 1: <8,16>, 2: <64,128>
+    int32_t S = 0, R = 0, C = 1;
+    uint32_t nreal = 0, ntiles = 0, ngroups = 0;
+    uint64_t total_counts = 0, total_sub = 0;
+    std::vector<RealInfo> info;
+    std::vector<RealCtl> ctl_host;
+    // device buffers
+    int32_t* d_net_i = nullptr;
+    double* d_net_d = nullptr;
+    RealInfo* d_info = nullptr;
+    RealCtl* d_ctl = nullptr;
+    TileEnt* d_tiles = nullptr;
+    uint32_t *d_x0 = nullptr, *d_x1 = nullptr;
+    uint32_t *d_rp = nullptr, *d_col = nullptr, *d_irp = nullptr, *d_icol = nullptr;
+    float *d_w = nullptr, *d_iw = nullptr;
+    int32_t* d_comp = nullptr;
+    PartA *d_pa_t = nullptr, *d_pa_g = nullptr;
+    PartB *d_pb_t = nullptr, *d_pb_g = nullptr;
+    uint32_t *d_cnt = nullptr;  // cntA_grp | cntA_real | cntB_grp | cntB_real | ndone
+    uint32_t* h_ndone = nullptr;  // pinned
+    double eps = 0.03;
+    int32_t nc = 10;
+    uint64_t seed = 0;
+    uint32_t wpb = 4, blocks = 8;
+    int use_graph = 1;
+    hipGraphExec_t graph = nullptr;
+    uint32_t graph_iters = 0;
+    std::vector<void*> allocs;
+};
+
+#define HIPCHK(ctx, call)                                                              \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess) {                                                        \
+            (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);            \
+            return PLIS_ERR_HIP;                                                       \
+        }                                                                              \
+    } while (0)
+
+template <typename T>
+static int dev_alloc(plis_ctx* c, T** p, size_t n) {
+    size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+    HIPCHK(c, hipMalloc((void**)p, bytes));
+    c->allocs.push_back(*p);
+    return 0;
+}
+
+template <typename T>
+static int upload(plis_ctx* c, T** p, const std::vector<T>& v) {
+    if (int rc = dev_alloc(c, p, v.size())) return rc;
+    if (!v.empty()) HIPCHK(c, hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return 0;
+}
+
+static int fail_create(const std::string& m) {
+    g_create_err = m;
+    return PLIS_ERR_ARG;
+}
+
+// ---------------------------------------------------------------------------
+// canonical CSR: self loops dropped, neighbours ascending, duplicate
+// neighbours merged (weights summed in double in input order, then rounded)
+// ---------------------------------------------------------------------------
+struct Csr {
+    std::vector<uint32_t> rp, col;
+    std::vector<float> w;
+};
+
+static bool canonical_csr(const plis_topology& t, Csr& out, std::string& err) {
+    const uint32_t N = t.n_subvolumes;
+    out.rp.assign(N + 1, 0);
+    out.col.clear();
+    out.w.clear();
+    out.col.reserve(t.n_edges);
+    out.w.reserve(t.n_edges);
+    struct Nb { uint32_t id, pos; float w; };
+    std::vector<Nb> tmp;
+    for (uint32_t v = 0; v < N; ++v) {
+        const uint32_t b = t.row_ptr[v], e = t.row_ptr[v + 1];
+        if (e < b || e > t.n_edges) { err = "row_ptr not monotone or beyond n_edges"; return false; }
+        tmp.clear();
+        for (uint32_t q = b; q < e; ++q) {
+            const uint32_t u = t.col[q];
+            if (u >= N) { err = "neighbour id >= n_subvolumes"; return false; }
+            if (u == v) continue;
+            const float wq = t.weight ? t.weight[q] : 1.0f;
+            if (!(wq > 0.0f) || !std::isfinite(wq)) { err = "edge weight must be positive and finite"; return false; }
+            tmp.push_back({u, q, wq});
+        }
+        std::sort(tmp.begin(), tmp.end(), [](const Nb& x, const Nb& y) {
+            return x.id != y.id ? x.id < y.id : x.pos < y.pos;
+        });
+        out.rp[v] = (uint32_t)out.col.size();
+        for (size_t i = 0; i < tmp.size();) {
+            size_t i2 = i;
+            double acc = 0.0;
+            while (i2 < tmp.size() && tmp[i2].id == tmp[i].id) acc += (double)tmp[i2++].w;
+            out.col.push_back(tmp[i].id);
+            out.w.push_back(i2 - i == 1 ? tmp[i].w : (float)acc);
+            i = i2;
+        }
+    }
+    out.rp[N] = (uint32_t)out.col.size();
+    return true;
+}
+
+static bool is_symmetric(const Csr& g, uint32_t N) {
+    for (uint32_t v = 0; v < N; ++v)
+        for (uint32_t q = g.rp[v]; q < g.rp[v + 1]; ++q) {
+            const uint32_t u = g.col[q];
+            auto b = g.col.begin() + g.rp[u], e = g.col.begin() + g.rp[u + 1];
+            auto it = std::lower_bound(b, e, v);
+            if (it == e || *it != v) return false;
+            if (g.w[it - g.col.begin()] != g.w[q]) return false;
+        }
+    return true;
+}
+
+static void transpose(const Csr& g, uint32_t N, Csr& in) {
+    in.rp.assign(N + 1, 0);
+    for (uint32_t q = 0; q < g.rp[N]; ++q) in.rp[g.col[q] + 1]++;
+    for (uint32_t v = 0; v < N; ++v) in.rp[v + 1] += in.rp[v];
+    in.col.assign(g.rp[N], 0);
+    in.w.assign(g.rp[N], 0.0f);
+    std::vector<uint32_t> fill(N, 0);
+    for (uint32_t u = 0; u < N; ++u)
+        for (uint32_t q = g.rp[u]; q < g.rp[u + 1]; ++q) {
+            const uint32_t v = g.col[q];
+            const uint32_t pos = in.rp[v] + fill[v]++;
+            in.col[pos] = u;
+            in.w[pos] = g.w[q];
+        }
+}
+
+// ---------------------------------------------------------------------------
+// network blob
+// ---------------------------------------------------------------------------
+static bool build_network(const plis_network& net, NetHeader& h, std::vector<int32_t>& I,
+                          std::vector<double>& D, std::string& err) {
+    const int S = net.n_species, R = net.n_reactions, C = net.n_compartments;
+    if (S < 1 || S > kMaxS) { err = "n_species must be in [1, 64]"; return false; }
+    if (R < 0 || R > kMaxR) { err = "n_reactions must be in [0, 128]"; return false; }
+    if (C < 1) { err = "n_compartments must be >= 1"; return false; }
+    if (!net.k || !net.d || (R > 0 && (!net.reactants || !net.products))) { err = "null network array"; return false; }
+    h = NetHeader{};
+    h.S = S; h.R = R; h.C = C;
+    I.clear();
+    D.clear();
+    auto put = [&](int32_t& off, size_t n) { off = (int32_t)I.size(); I.resize(I.size() + std::max<size_t>(n, 1), 0); };
+    put(h.off_nu, (size_t)S * R);
+    put(h.off_rs, (size_t)R * 3);
+    put(h.off_rr, (size_t)R * 3);
+    put(h.off_nrs, R);
+    put(h.off_order, R);
+    put(h.off_hor, S);
+    put(h.off_mult, S);
+    for (int j = 0; j < R; ++j) {
+        int ord = 0, n = 0;
+        for (int s = 0; s < S; ++s) {
+            const int r = net.reactants[s * R + j], p = net.products[s * R + j];
+            if (r < 0 || p < 0) { err = "negative stoichiometric coefficient"; return false; }
+            I[h.off_nu + s * R + j] = p - r;
+            if (r > 0) {
+                if (n >= 3) { err = "reaction order above 3"; return false; }
+                I[h.off_rs + j * 3 + n] = s;
+                I[h.off_rr + j * 3 + n] = r;
+                ++n;
+                ord += r;
+            }
+        }
+        if (ord > 3) { err = "reaction order above 3"; return false; }
+        I[h.off_nrs + j] = n;
+        I[h.off_order + j] = ord;
+    }
+    for (int s = 0; s < S; ++s) {
+        int hor = 0, mult = 0;
+        for (int j = 0; j < R; ++j) {
+            const int r = net.reactants[s * R + j];
+            if (r <= 0) continue;
+            const int o = I[h.off_order + j];
+            if (o > hor) { hor = o; mult = r; }
+            else if (o == hor && r > mult) mult = r;
+        }
+        bool df = false;
+        for (int c = 0; c < C; ++c) df = df || net.d[c * S + s] > 0.0;
+        if (df && hor == 0) { hor = 1; mult = 1; }
+        I[h.off_hor + s] = hor;
+        I[h.off_mult + s] = mult;
+    }
+    h.ints_total = (int32_t)I.size();
+    h.off_k = 0;
+    D.assign(net.k, net.k + (size_t)C * R);
+    h.off_d = (int32_t)D.size();
+    D.insert(D.end(), net.d, net.d + (size_t)C * S);
+    for (double v : D)
+        if (!(v >= 0.0) || !std::isfinite(v)) { err = "rate constants must be finite and >= 0"; return false; }
+    if (D.empty()) D.push_back(0.0);
+    h.dbl_total = (int32_t)D.size();
+    return true;
+}
+
+static KernelArgs make_args(plis_ctx* c) {
+    KernelArgs a{};
+    a.net_i = c->d_net_i;
+    a.net_d = c->d_net_d;
+    a.nh = c->nh;
+    a.info = c->d_info;
+    a.ctl = c->d_ctl;
+    a.tiles = c->d_tiles;
+    a.ntiles = c->ntiles;
+    a.nreal = c->nreal;
+    a.x0 = c->d_x0;
+    a.x1 = c->d_x1;
+    a.rp = c->d_rp;
+    a.col = c->d_col;
+    a.w = c->d_w;
+    a.irp = c->d_irp;
+    a.icol = c->d_icol;
+    a.iw = c->d_iw;
+    a.comp = c->d_comp;
+    a.partA_tile = c->d_pa_t;
+    a.partA_grp = c->d_pa_g;
+    a.partB_tile = c->d_pb_t;
+    a.partB_grp = c->d_pb_g;
+    a.cntA_grp = c->d_cnt;
+    a.cntA_real = c->d_cnt + c->ngroups;
+    a.cntB_grp = c->d_cnt + c->ngroups + c->nreal;
+    a.cntB_real = c->d_cnt + 2 * c->ngroups + c->nreal;
+    a.ndone = c->d_cnt + 2 * c->ngroups + 2 * c->nreal;
+    a.epsilon = c->eps;
+    a.n_critical = c->nc;
+    a.seed_lo = (uint32_t)c->seed;
+    a.seed_hi = (uint32_t)(c->seed >> 32);
+    a.waves_per_block = c->wpb;
+    return a;
+}
+
+static void launch_phase(plis_ctx* c, int phase, const KernelArgs& a) {
+    const dim3 grid(c->blocks), block(64 * c->wpb);
+    switch (c->variant * 2 + phase) {
+        case 0: hipLaunchKernelGGL((phase_a_kernel<1, 4>), grid, block, 0, c->stream, a); break;
+        case 1: hipLaunchKernelGGL((phase_b_kernel<1, 4>), grid, block, 0, c->stream, a); break;
+        case 2: hipLaunchKernelGGL((phase_a_kernel<8, 16>), grid, block, 0, c->stream, a); break;
+        case 3: hipLaunchKernelGGL((phase_b_kernel<8, 16>), grid, block, 0, c->stream, a); break;
+        case 4: hipLaunchKernelGGL((phase_a_kernel<64, 128>), grid, block, 0, c->stream, a); break;
+        default: hipLaunchKernelGGL((phase_b_kernel<64, 128>), grid, block, 0, c->stream, a); break;
+    }
+}
+
+static void set_shape(plis_ctx* c, uint32_t wpb, uint32_t bpc) {
+    c->wpb = std::max(1u, std::min(4u, wpb));
+    const uint32_t need = (c->ntiles + c->wpb - 1) / c->wpb;
+    uint32_t blocks = std::min<uint32_t>((uint32_t)c->cus * std::max(1u, bpc), need);
+    blocks = std::max(8u, (blocks + 7u) & ~7u);
+    c->blocks = blocks;
+    if (c->graph) {
+        hipGraphExecDestroy(c->graph);
+        c->graph = nullptr;
+        c->graph_iters = 0;
+    }
+}
+
+static int launch_iters(plis_ctx* c, uint32_t n) {
+    const KernelArgs a = make_args(c);
+    for (uint32_t i = 0; i < n; ++i) {
+        launch_phase(c, 0, a);
+        launch_phase(c, 1, a);
+    }
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
+static int graph_iters(plis_ctx* c, uint32_t n) {
+    if (!c->use_graph) return launch_iters(c, n);
+    if (c->graph_iters != n) {
+        if (c->graph) {
+            hipGraphExecDestroy(c->graph);
+            c->graph = nullptr;
+        }
+        hipGraph_t g;
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        const KernelArgs a = make_args(c);
+        for (uint32_t i = 0; i < n; ++i) {
+            launch_phase(c, 0, a);
+            launch_phase(c, 1, a);
+        }
+        HIPCHK(c, hipStreamEndCapture(c->stream, &g));
+        HIPCHK(c, hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0));
+        HIPCHK(c, hipGraphDestroy(g));
+        c->graph_iters = n;
+    }
+    HIPCHK(c, hipGraphLaunch(c->graph, c->stream));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" int plis_device_count(char* name, int32_t name_len) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    if (name && name_len > 0) {
+        name[0] = 0;
+        hipDeviceProp_t p;
+        if (n > 0 && hipGetDeviceProperties(&p, 0) == hipSuccess) {
+            std::snprintf(name, (size_t)name_len, "%s (%s)", p.name, p.gcnArchName);
+        }
+    }
+    return n;
+}
+
+extern "C" const char* plis_last_error(const plis_ctx* ctx) {
+    return ctx ? ctx->err.c_str() : g_create_err.c_str();
+}
+
+extern "C" void plis_destroy(plis_ctx* c) {
+    if (!c) return;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->graph) hipGraphExecDestroy(c->graph);
+    for (void* p : c->allocs) hipFree(p);
+    if (c->h_ndone) hipHostFree(c->h_ndone);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+extern "C" int plis_create(const plis_network* net, int32_t n_real, const plis_topology* topos,
+                           const uint32_t* const* x0, const uint32_t* ids, const plis_params* params,
+                           plis_ctx** out) {
+    if (!out) return fail_create("out is NULL");
+    *out = nullptr;
+    if (!net || !topos || !x0 || !params || n_real < 1) return fail_create("null argument or n_real < 1");
+    if (!(params->epsilon > 0.0) || params->n_critical < 0) return fail_create("epsilon must be > 0, n_critical >= 0");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        g_create_err = "no HIP device available";
+        return PLIS_ERR_NODEV;
+    }
+    NetHeader nh;
+    std::vector<int32_t> I;
+    std::vector<double> D;
+    std::string err;
+    if (!build_network(*net, nh, I, D, err)) return fail_create(err);
+    const int S = nh.S;
+
+    plis_ctx* c = new plis_ctx();
+    c->nh = nh;
+    c->S = nh.S;
+    c->R = nh.R;
+    c->C = nh.C;
+    c->variant = (nh.S <= 1 && nh.R <= 4) ? 0 : (nh.S <= 8 && nh.R <= 16) ? 1 : 2;
+    c->eps = params->epsilon;
+    c->nc = params->n_critical;
+    c->seed = params->seed;
+    c->use_graph = params->use_graph;
+    c->nreal = (uint32_t)n_real;
+
+    // host-side layout
+    std::vector<uint32_t> hx, hrp, hcol, hirp, hicol;
+    std::vector<float> hw, hiw;
+    std::vector<int32_t> hcomp;
+    std::vector<TileEnt> tiles;
+    bool any_weighted = false, any_asym = false;
+    uint64_t xoff = 0, eoff = 0, rpoff = 0, ieoff = 0, irpoff = 0, coff = 0;
+    uint32_t tile = 0, group = 0;
+    for (int32_t r = 0; r < n_real; ++r) {
+        const plis_topology& t = topos[r];
+        const uint32_t N = t.n_subvolumes;
+        const uint32_t id = ids ? ids[r] : (uint32_t)r;
+        if (N < 1 || !t.row_ptr || (t.n_edges && !t.col) || !x0[r]) {
+            delete c;
+            return fail_create("realization " + std::to_string(r) + ": empty topology or null array");
+        }
+        if (id > 0xFFFFu) { delete c; return fail_create("realization id must be < 65536"); }
+        Csr g;
+        if (!canonical_csr(t, g, err)) { delete c; return fail_create("realization " + std::to_string(r) + ": " + err); }
+        bool weighted = false;
+        for (float wv : g.w) weighted = weighted || wv != 1.0f;
+        const bool sym = is_symmetric(g, N);
+        RealInfo ri{};
+        ri.x_off = xoff;
+        ri.e_off = eoff;
+        ri.rp_off = rpoff;
+        ri.N = N;
+        ri.id = id;
+        ri.flags = (sym ? 1u : 0u) | (weighted ? 2u : 0u);
+        any_weighted = any_weighted || weighted;
+        hrp.insert(hrp.end(), g.rp.begin(), g.rp.end());
+        hcol.insert(hcol.end(), g.col.begin(), g.col.end());
+        hw.insert(hw.end(), g.w.begin(), g.w.end());
+        if (!sym) {
+            any_asym = true;
+            Csr in;
+            transpose(g, N, in);
+            ri.ie_off = ieoff;
+            ri.irp_off = irpoff;
+            hirp.insert(hirp.end(), in.rp.begin(), in.rp.end());
+            hicol.insert(hicol.end(), in.col.begin(), in.col.end());
+            hiw.insert(hiw.end(), in.w.begin(), in.w.end());
+            ieoff += in.col.size();
+            irpoff += N + 1;
+        } else {
+            ri.ie_off = eoff;
+            ri.irp_off = rpoff;
+        }
+        if (nh.C > 1) {
+            ri.comp_off = coff;
+            for (uint32_t v = 0; v < N; ++v) {
+                const int32_t cv = t.compartment ? t.compartment[v] : 0;
+                if (cv < 0 || cv >= nh.C) { delete c; return fail_create("compartment id out of range"); }
+                hcomp.push_back(cv);
+            }
+            coff += N;
+        }
+        hx.insert(hx.end(), x0[r], x0[r] + (size_t)S * N);
+        ri.tile_begin = tile;
+        ri.ntiles = (N + kTile - 1) / kTile;
+        ri.group_begin = group;
+        ri.ngroups = (ri.ntiles + kGroup - 1) / kGroup;
+        for (uint32_t q = 0; q < ri.ntiles; ++q) tiles.push_back(TileEnt{(uint32_t)r, q * (uint32_t)kTile});
+        tile += ri.ntiles;
+        group += ri.ngroups;
+        xoff += (uint64_t)S * N;
+        eoff += g.col.size();
+        rpoff += N + 1;
+        c->info.push_back(ri);
+        c->total_sub += N;
+    }
+    c->ntiles = tile;
+    c->ngroups = group;
+    c->total_counts = xoff;
+
+    hipSetDevice(0);
+    c->device = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, 0) != hipSuccess) { delete c; g_create_err = "hipGetDeviceProperties failed"; return PLIS_ERR_HIP; }
+    c->cus = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        g_create_err = "hipStreamCreate failed";
+        return PLIS_ERR_HIP;
+    }
+    c->ctl_host.resize(n_real);
+    for (auto& rc : c->ctl_host) {
+        rc = RealCtl{};
+        rc.t_end = __builtin_huge_val();
+        rc.t_pause = __builtin_huge_val();
+        rc.done = 1u;
+        rc.ev_idx = -1;
+    }
+    int rc = 0;
+    std::vector<uint32_t> zeros_cnt(2 * (size_t)c->ngroups + 2 * (size_t)c->nreal + 1, 0u);
+    if ((rc = upload(c, &c->d_net_i, I)) || (rc = upload(c, &c->d_net_d, D)) ||
+        (rc = upload(c, &c->d_info, c->info)) || (rc = upload(c, &c->d_ctl, c->ctl_host)) ||
+        (rc = upload(c, &c->d_tiles, tiles)) || (rc = upload(c, &c->d_x0, hx)) ||
+        (rc = dev_alloc(c, &c->d_x1, hx.size())) || (rc = upload(c, &c->d_rp, hrp)) ||
+        (rc = upload(c, &c->d_col, hcol)) ||
+        (rc = upload(c, &c->d_w, any_weighted ? hw : std::vector<float>())) ||
+        (rc = upload(c, &c->d_irp, hirp)) || (rc = upload(c, &c->d_icol, hicol)) ||
+        (rc = upload(c, &c->d_iw, hiw)) || (rc = upload(c, &c->d_comp, hcomp)) ||
+        (rc = dev_alloc(c, &c->d_pa_t, c->ntiles)) || (rc = dev_alloc(c, &c->d_pa_g, c->ngroups)) ||
+        (rc = dev_alloc(c, &c->d_pb_t, c->ntiles)) || (rc = dev_alloc(c, &c->d_pb_g, c->ngroups)) ||
+        (rc = upload(c, &c->d_cnt, zeros_cnt))) {
+        g_create_err = c->err;
+        plis_destroy(c);
+        return rc;
+    }
+    (void)any_asym;
+    if (hipHostMalloc((void**)&c->h_ndone, sizeof(uint32_t)) != hipSuccess) {
+        g_create_err = "hipHostMalloc failed";
+        plis_destroy(c);
+        return PLIS_ERR_HIP;
+    }
+    set_shape(c, params->waves_per_block > 0 ? (uint32_t)params->waves_per_block : 4u,
+              params->blocks_per_cu > 0 ? (uint32_t)params->blocks_per_cu : 4u);
+    *out = c;
+    return PLIS_OK;
+}
+
+static int pull_ctl(plis_ctx* c) {
+    HIPCHK(c, hipMemcpyAsync(c->ctl_host.data(), c->d_ctl, c->ctl_host.size() * sizeof(RealCtl),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// Arm every realization for (t_end, t_pause, step_limit); returns how many
+// are already done through *ndone_out.
+static int arm(plis_ctx* c, double t_end, double t_pause, uint64_t step_limit, uint32_t* ndone_out) {
+    if (int rc = pull_ctl(c)) return rc;
+    uint32_t ndone = 0;
+    for (auto& r : c->ctl_host) {
+        r.t_end = t_end;
+        r.t_pause = t_pause;
+        r.step_limit = step_limit;
+        const bool stop = r.status == kAbsorbed || r.status < 0 || r.t >= t_end || r.steps >= step_limit ||
+                          r.t >= t_pause;
+        r.done = stop ? 1u : 0u;
+        if (!stop) r.status = kRunning;
+        else if (r.status == kRunning && (r.t >= t_end || r.steps >= step_limit)) r.status = kDone;
+        ndone += r.done;
+    }
+    HIPCHK(c, hipMemcpyAsync(c->d_ctl, c->ctl_host.data(), c->ctl_host.size() * sizeof(RealCtl),
+                             hipMemcpyHostToDevice, c->stream));
+    uint32_t* d_ndone = c->d_cnt + 2 * c->ngroups + 2 * c->nreal;
+    HIPCHK(c, hipMemcpyAsync(d_ndone, &ndone, sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *ndone_out = ndone;
+    return 0;
+}
+
+extern "C" int plis_arm(plis_ctx* c, double t_end, double t_pause, uint64_t step_limit) {
+    if (!c) return PLIS_ERR_ARG;
+    uint32_t nd = 0;
+    return arm(c, t_end, t_pause, step_limit, &nd);
+}
+
+extern "C" int plis_run_until(plis_ctx* c, double t_end, double t_pause, uint64_t step_limit) {
+    if (!c) return PLIS_ERR_ARG;
+    uint32_t ndone = 0;
+    if (int rc = arm(c, t_end, t_pause, step_limit, &ndone)) return rc;
+    uint32_t* d_ndone = c->d_cnt + 2 * c->ngroups + 2 * c->nreal;
+    uint32_t k = 2;
+    uint64_t polls = 0, last_progress = 0;
+    auto progress = [&]() {
+        uint64_t p = 0;
+        for (const auto& r : c->ctl_host) p += r.steps + r.leapbacks;
+        return p;
+    };
+    last_progress = progress();
+    while (ndone < c->nreal) {
+        if (int rc = graph_iters(c, k)) return rc;
+        HIPCHK(c, hipMemcpyAsync(c->h_ndone, d_ndone, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        ndone = *c->h_ndone;
+        if (k < 64) k *= 2;
+        // every iteration of an armed realization either accepts a step or
+        // leaps back; a batch of iterations without either means the control
+        // state is inconsistent, so stop instead of spinning
+        if (++polls % 8 == 0 && ndone < c->nreal) {
+            if (int rc = pull_ctl(c)) return rc;
+            const uint64_t p = progress();
+            if (p == last_progress) {
+                c->err = "no realization advanced over a batch of iterations";
+                return PLIS_ERR_STATE;
+            }
+            last_progress = p;
+        }
+    }
+    if (int rc = pull_ctl(c)) return rc;
+    for (auto& r : c->ctl_host)
+        if (r.status == kErrLeapback) {
+            c->err = "a step needed more than 63 leap-backs";
+            return PLIS_ERR_LEAPBACK;
+        }
+    return PLIS_OK;
+}
+
+extern "C" int plis_run(plis_ctx* c, double t_end, uint64_t step_limit) {
+    return plis_run_until(c, t_end, __builtin_huge_val(), step_limit);
+}
+
+extern "C" int plis_iterate(plis_ctx* c, uint32_t n) {
+    if (!c) return PLIS_ERR_ARG;
+    while (n > 0) {
+        const uint32_t k = std::min<uint32_t>(n, 64u);
+        if (int rc = (k == 64u ? graph_iters(c, k) : launch_iters(c, k))) return rc;
+        n -= k;
+    }
+    return PLIS_OK;
+}
+
+extern "C" int plis_profile(plis_ctx* c, uint32_t n, double* out) {
+    if (!c || !out) return PLIS_ERR_ARG;
+    std::vector<hipEvent_t> ev(4 * (size_t)n + 2);
+    for (auto& e : ev) HIPCHK(c, hipEventCreate(&e));
+    const KernelArgs a = make_args(c);
+    HIPCHK(c, hipEventRecord(ev[0], c->stream));
+    for (uint32_t i = 0; i < n; ++i) {
+        HIPCHK(c, hipEventRecord(ev[1 + 4 * i], c->stream));
+        launch_phase(c, 0, a);
+        HIPCHK(c, hipEventRecord(ev[2 + 4 * i], c->stream));
+        HIPCHK(c, hipEventRecord(ev[3 + 4 * i], c->stream));
+        launch_phase(c, 1, a);
+        HIPCHK(c, hipEventRecord(ev[4 + 4 * i], c->stream));
+    }
+    HIPCHK(c, hipEventRecord(ev[4 * n + 1], c->stream));
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventSynchronize(ev[4 * n + 1]));
+    float tot = 0.f;
+    double ta = 0.0, tb = 0.0;
+    HIPCHK(c, hipEventElapsedTime(&tot, ev[0], ev[4 * n + 1]));
+    for (uint32_t i = 0; i < n; ++i) {
+        float x = 0.f, y = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&x, ev[1 + 4 * i], ev[2 + 4 * i]));
+        HIPCHK(c, hipEventElapsedTime(&y, ev[3 + 4 * i], ev[4 + 4 * i]));
+        ta += x;
+        tb += y;
+    }
+    for (auto& e : ev) hipEventDestroy(e);
+    out[0] = tot;
+    out[1] = ta;
+    out[2] = tb;
+    return PLIS_OK;
+}
+
+extern "C" int plis_sync(plis_ctx* c) {
+    if (!c) return PLIS_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PLIS_OK;
+}
+
+extern "C" int plis_set_state(plis_ctx* c, int32_t r, const uint32_t* x) {
+    if (!c || !x || r < 0 || (uint32_t)r >= c->nreal) return PLIS_ERR_ARG;
+    if (int rc = pull_ctl(c)) return rc;
+    const RealInfo& ri = c->info[r];
+    uint32_t* X = c->ctl_host[r].cur ? c->d_x1 : c->d_x0;
+    HIPCHK(c, hipMemcpy(X + ri.x_off, x, (size_t)c->S * ri.N * sizeof(uint32_t), hipMemcpyHostToDevice));
+    return PLIS_OK;
+}
+
+extern "C" int plis_get_state(plis_ctx* c, int32_t r, uint32_t* x) {
+    if (!c || !x || r < 0 || (uint32_t)r >= c->nreal) return PLIS_ERR_ARG;
+    if (int rc = pull_ctl(c)) return rc;
+    const RealInfo& ri = c->info[r];
+    const uint32_t* X = c->ctl_host[r].cur ? c->d_x1 : c->d_x0;
+    HIPCHK(c, hipMemcpy(x, X + ri.x_off, (size_t)c->S * ri.N * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    return PLIS_OK;
+}
+
+extern "C" int plis_get_stats(plis_ctx* c, int32_t r, plis_stats* st) {
+    if (!c || !st || r < 0 || (uint32_t)r >= c->nreal) return PLIS_ERR_ARG;
+    if (int rc = pull_ctl(c)) return rc;
+    const RealCtl& q = c->ctl_host[r];
+    st->t = q.t;
+    st->tau = q.tau;
+    st->steps = q.steps;
+    st->leapbacks = q.leapbacks;
+    st->firings = q.firings;
+    st->subvolume_leaps = q.subvolume_leaps;
+    st->crit_events = q.crit_events;
+    st->status = q.status;
+    st->pad = 0;
+    return PLIS_OK;
+}
+
+extern "C" int plis_debug_tau(plis_ctx* c, double* tp, double* a0c, double* t2) {
+    if (!c || !tp || !a0c || !t2) return PLIS_ERR_ARG;
+    double *d_tp, *d_a0c, *d_t2;
+    const size_t n = c->total_sub;
+    HIPCHK(c, hipMalloc((void**)&d_tp, n * sizeof(double)));
+    HIPCHK(c, hipMalloc((void**)&d_a0c, n * sizeof(double)));
+    HIPCHK(c, hipMalloc((void**)&d_t2, n * sizeof(double)));
+    KernelArgs a = make_args(c);
+    a.dbg_tp = d_tp;
+    a.dbg_a0c = d_a0c;
+    a.dbg_t2 = d_t2;
+    launch_phase(c, 0, a);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipMemcpy(tp, d_tp, n * sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(a0c, d_a0c, n * sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(t2, d_t2, n * sizeof(double), hipMemcpyDeviceToHost);
+    hipFree(d_tp);
+    hipFree(d_a0c);
+    hipFree(d_t2);
+    if (e != hipSuccess) {
+        c->err = std::string("plis_debug_tau: ") + hipGetErrorString(e);
+        return PLIS_ERR_HIP;
+    }
+    return PLIS_OK;
+}
+
+extern "C" int plis_launch_shape(plis_ctx* c, int32_t* out) {
+    if (!c || !out) return PLIS_ERR_ARG;
+    out[0] = (int32_t)c->wpb;
+    out[1] = (int32_t)c->blocks;
+    out[2] = (int32_t)c->ntiles;
+    return PLIS_OK;
+}
+
+extern "C" int plis_autotune(plis_ctx* c, uint32_t iters) {
+    if (!c) return PLIS_ERR_ARG;
+    if (iters < 1) iters = 1;
+    const uint32_t wpbs[] = {1, 2, 4};
+    const uint32_t bpcs[] = {1, 2, 4, 8, 16};
+    double best = 1e300;
+    uint32_t bw = c->wpb, bb = 4;
+    std::vector<double> prof(3);
+    for (uint32_t w : wpbs)
+        for (uint32_t b : bpcs) {
+            set_shape(c, w, b);
+            if (int rc = plis_profile(c, 1, prof.data())) return rc;  // warm
+            if (int rc = plis_profile(c, iters, prof.data())) return rc;
+            if (prof[1] + prof[2] < best) {
+                best = prof[1] + prof[2];
+                bw = w;
+                bb = b;
+            }
+        }
+    set_shape(c, bw, bb);
+    return PLIS_OK;
+}

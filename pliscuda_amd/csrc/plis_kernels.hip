@@ -1,0 +1,739 @@
+// plis_kernels.hip -- gfx950 kernels for the RDME tau-leap + SSA step.
+//
+// One global step of one realization is two persistent kernels:
+//   phase A (tau): per subvolume FP64 mass-action propensities, Cao tau' with
+//     hops treated as first-order channels, the critical set and an
+//     exponential tau'' clock; a wave-shuffle min/argmin per 64-subvolume
+//     tile, then a last-arriver reduction over groups of 64 tiles and over the
+//     realization's groups.  The last wave decides the step (tau, critical
+//     event) for its realization.  Replaces compute_rates
+//     (PLISCuda/src/rates.cu:82-96), compute_taus (tau.cu:197-251) and the
+//     host thrust::min_element + tau[i] round trips (driver.cu:267-277).
+//   phase B (leap): per subvolume Poisson firings of non-critical reactions,
+//     gather-form diffusion (each subvolume regenerates its in-neighbours'
+//     hop counts from the same counter-based Philox draw the sender used, so
+//     there are no atomics on counts), the critical event of the argmin
+//     subvolume, and the negative-population check; the last wave commits the
+//     step or leaps back (tau' halved, new attempt index).  Replaces
+//     leap_step (leap.cu:3-201), ssa_step (ssa.cu:56-110), check_state +
+//     none_of + revert (leap.cu:203-212, driver.cu:292-323).
+//
+// Wavefront-per-tile: lane i of a wave owns subvolume v0+i of its tile;
+// species-major counts make every per-species load and store one coalesced
+// 256-byte access.  Blocks are dispatched round-robin over the 8 XCDs, so
+// each XCD walks a contiguous eighth of the tile list (its L2 holds the
+// neighbouring tiles it gathers from).
+#include "plis_numerics.h"
+#include "plis_types.h"
+
+namespace plis {
+
+constexpr double kInf = __builtin_huge_val();
+
+struct NetView {
+    const int32_t *nu, *rs, *rr, *nrs, *order, *hor, *mult;
+    const double *k, *d;
+    int S, R, C;
+};
+
+__device__ __forceinline__ NetView net_view(const KernelArgs& a) {
+    NetView n;
+    n.nu = a.net_i + a.nh.off_nu;
+    n.rs = a.net_i + a.nh.off_rs;
+    n.rr = a.net_i + a.nh.off_rr;
+    n.nrs = a.net_i + a.nh.off_nrs;
+    n.order = a.net_i + a.nh.off_order;
+    n.hor = a.net_i + a.nh.off_hor;
+    n.mult = a.net_i + a.nh.off_mult;
+    n.k = a.net_d + a.nh.off_k;
+    n.d = a.net_d + a.nh.off_d;
+    n.S = a.nh.S;
+    n.R = a.nh.R;
+    n.C = a.nh.C;
+    return n;
+}
+
+__device__ __forceinline__ uint32_t wave_uniform(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+}
+
+// XCD-aware persistent tile walk (grid is a multiple of 8 blocks).
+__device__ __forceinline__ uint32_t tile_at(uint32_t it, uint32_t ntiles, uint32_t wpb) {
+    const uint32_t G = gridDim.x;
+    const uint32_t b = blockIdx.x;
+    const uint32_t wv = wave_uniform(threadIdx.x >> 6);
+    const uint32_t xcd = b & 7u;
+    const uint32_t nbx = (G - xcd + 7u) >> 3;
+    const uint32_t lb = b >> 3;
+    const uint32_t cb = (uint32_t)(((uint64_t)ntiles * xcd) >> 3);
+    const uint32_t ce = (uint32_t)(((uint64_t)ntiles * (xcd + 1u)) >> 3);
+    const uint32_t t = cb + lb * wpb + wv + it * nbx * wpb;
+    return t < ce ? t : 0xFFFFFFFFu;
+}
+
+__device__ __forceinline__ double binom_d(uint32_t x, int r) {
+    if ((long long)x < (long long)r) return 0.0;
+    const double xd = (double)x;
+    if (r == 1) return xd;
+    if (r == 2) return (xd * (xd - 1.0)) * 0.5;
+    return ((xd * (xd - 1.0)) * (xd - 2.0)) / 6.0;
+}
+
+__device__ __forceinline__ bool hop_active(double d, uint32_t x, int nc) {
+    return d > 0.0 && (long long)x >= (long long)nc && x > 0u;
+}
+
+__device__ __forceinline__ double g_cao(int hor, int mult, uint32_t xs) {
+    const double xd = (double)xs;
+    if (hor == 1) return 1.0;
+    if (hor == 2) return mult == 2 ? 2.0 + 1.0 / (xd - 1.0) : 2.0;
+    if (mult == 3) return (3.0 + 1.0 / (xd - 1.0)) + 2.0 / (xd - 2.0);
+    if (mult == 2) return 1.5 * (2.0 + 1.0 / (xd - 1.0));
+    return 3.0;
+}
+
+// Reaction propensities a_j = k_j prod_s C(x_s, r_sj) and the critical flags
+// (Cao: L_j = min_s floor(x_s / r_sj) < n_c) at one subvolume.
+template <int MR, int MS>
+__device__ __forceinline__ void react_props(const NetView& n, const uint32_t (&x)[MS], int c, int nc,
+                                            double (&av)[MR], uint32_t (&crit)[(MR + 31) / 32]) {
+#pragma unroll
+    for (int i = 0; i < (MR + 31) / 32; ++i) crit[i] = 0u;
+#pragma unroll
+    for (int j = 0; j < MR; ++j) {
+        if (j >= n.R) break;
+        double aj = n.k[c * n.R + j];
+        long long L = -1;
+        const int nr = n.nrs[j];
+        for (int i = 0; i < nr; ++i) {
+            const int s = n.rs[j * 3 + i], r = n.rr[j * 3 + i];
+            uint32_t xs = 0;
+#pragma unroll
+            for (int q = 0; q < MS; ++q)
+                if (q == s) xs = x[q];
+            aj = aj * binom_d(xs, r);
+            const long long l = (long long)(xs / (uint32_t)r);
+            if (L < 0 || l < L) L = l;
+        }
+        av[j] = aj;
+        if (n.order[j] > 0 && L < (long long)nc) crit[j >> 5] |= 1u << (j & 31);
+    }
+}
+
+template <int MR>
+__device__ __forceinline__ bool is_crit(const uint32_t (&crit)[(MR + 31) / 32], int j) {
+    return (crit[j >> 5] >> (j & 31)) & 1u;
+}
+
+// ---------------------------------------------------------------------------
+// per-realization step decision (tau = min(tau', tau''), clamp to t_end)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void decide(RealCtl& c) {
+    double tau = fmin(c.tau_p, c.tau2);
+    c.crit = (c.tau2 <= c.tau_p) ? 1 : 0;
+    c.last = 0;
+    if (c.t + tau >= c.t_end) {
+        tau = c.t_end - c.t;
+        c.crit = 0;
+        c.last = 1;
+    }
+    c.tau = tau;
+}
+
+// Critical event at v*: a critical reaction or a critical hop, chosen with
+// probability proportional to its propensity; a hop's edge is chosen in
+// proportion to its weight.  (The reference picks a uniform slot among 6
+// including self-padding, PLISCuda/src/leap.cu:180-194, ssa.cu:88-106.)
+template <int MS, int MR>
+__device__ void choose_event(const KernelArgs& a, const NetView& n, const RealInfo& ri, RealCtl& c,
+                             const uint32_t* X) {
+    const uint32_t v = c.vstar;
+    const uint32_t N = ri.N;
+    uint32_t x[MS];
+#pragma unroll
+    for (int s = 0; s < MS; ++s) x[s] = s < n.S ? X[ri.x_off + (uint64_t)s * N + v] : 0u;
+    const int cc = n.C > 1 ? a.comp[ri.comp_off + v] : 0;
+    double av[MR];
+    uint32_t crit[(MR + 31) / 32];
+    react_props<MR, MS>(n, x, cc, a.n_critical, av, crit);
+    const uint32_t rb = a.rp[ri.rp_off + v], re = a.rp[ri.rp_off + v + 1];
+    const bool weighted = (ri.flags & 2u) != 0;
+    double W = 0.0;
+    for (uint32_t q = rb; q < re; ++q) W += weighted ? (double)a.w[ri.e_off + q] : 1.0;
+    double a0c = 0.0;
+#pragma unroll
+    for (int j = 0; j < MR; ++j)
+        if (j < n.R && is_crit<MR>(crit, j)) a0c += av[j];
+#pragma unroll
+    for (int s = 0; s < MS; ++s) {
+        if (s >= n.S) break;
+        const double d = n.d[cc * n.S + s];
+        if (d > 0.0 && !hop_active(d, x[s], a.n_critical)) a0c += (d * (double)x[s]) * W;
+    }
+    const U4 o = philox10(v, kSelfId, c.step, ctr_word3(ri.id, 0u, kCrit, 0u), a.seed_lo, a.seed_hi);
+    const double target = uniform01(o.x) * a0c;
+    double acc = 0.0;
+    int kind = 0, idx = -1;
+    for (int j = 0; j < n.R; ++j) {
+        if (!is_crit<MR>(crit, j) || !(av[j] > 0.0)) continue;
+        acc += av[j];
+        idx = j;
+        if (target < acc) { kind = 1; break; }
+    }
+    int lastkind = idx >= 0 ? 1 : 0;
+    if (kind == 0) {
+        for (int s = 0; s < n.S; ++s) {
+            const double d = n.d[cc * n.S + s];
+            if (!(d > 0.0) || hop_active(d, x[s], a.n_critical)) continue;
+            const double ao = (d * (double)x[s]) * W;
+            if (!(ao > 0.0)) continue;
+            acc += ao;
+            idx = s;
+            lastkind = 2;
+            if (target < acc) { kind = 2; break; }
+        }
+    }
+    if (kind == 0) kind = lastkind;
+    c.ev_kind = kind;
+    c.ev_idx = idx;
+    c.ev_dst = v;
+    if (kind == 2) {
+        const double tw = uniform01(o.y) * W;
+        double accw = 0.0;
+        uint32_t dst = v;
+        for (uint32_t q = rb; q < re; ++q) {
+            accw += weighted ? (double)a.w[ri.e_off + q] : 1.0;
+            dst = a.col[ri.e_off + q];
+            if (tw < accw) break;
+        }
+        c.ev_dst = dst;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// phase A: tau' / tau'' per subvolume, min / argmin per realization
+// ---------------------------------------------------------------------------
+template <int MS, int MR>
+__device__ __forceinline__ void tau_subvolume(const KernelArgs& a, const NetView& n, const RealInfo& ri,
+                                              const uint32_t* X, uint32_t v, double& tp_out,
+                                              double& a0c_out) {
+    const uint32_t N = ri.N;
+    const int nc = a.n_critical;
+    uint32_t x[MS];
+#pragma unroll
+    for (int s = 0; s < MS; ++s) x[s] = s < n.S ? X[ri.x_off + (uint64_t)s * N + v] : 0u;
+    const int c = n.C > 1 ? a.comp[ri.comp_off + v] : 0;
+    double av[MR];
+    uint32_t crit[(MR + 31) / 32];
+    react_props<MR, MS>(n, x, c, nc, av, crit);
+
+    const bool weighted = (ri.flags & 2u) != 0;
+    const uint32_t rb = a.rp[ri.rp_off + v], re = a.rp[ri.rp_off + v + 1];
+    double W = 0.0;
+    if (weighted) {
+        for (uint32_t q = rb; q < re; ++q) W += (double)a.w[ri.e_off + q];
+    } else {
+        W = (double)(re - rb);
+    }
+    double a0c = 0.0;
+#pragma unroll
+    for (int j = 0; j < MR; ++j)
+        if (j < n.R && is_crit<MR>(crit, j)) a0c += av[j];
+
+    double mu[MS], sig[MS];
+#pragma unroll
+    for (int s = 0; s < MS; ++s) {
+        mu[s] = 0.0;
+        sig[s] = 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < MR; ++j) {
+        if (j >= n.R) break;
+        if (is_crit<MR>(crit, j)) continue;
+#pragma unroll
+        for (int s = 0; s < MS; ++s) {
+            if (s >= n.S) break;
+            const int nu = n.nu[s * n.R + j];
+            if (nu == 0) continue;
+            mu[s] += (double)nu * av[j];
+            sig[s] += (double)(nu * nu) * av[j];
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < MS; ++s) {
+        if (s >= n.S) break;
+        const double d = n.d[c * n.S + s];
+        if (d > 0.0 && !hop_active(d, x[s], nc)) a0c += (d * (double)x[s]) * W;
+        if (hop_active(d, x[s], nc)) {
+            const double aout = (d * (double)x[s]) * W;
+            mu[s] -= aout;
+            sig[s] += aout;
+        }
+    }
+    // in-hops (nu = +1) from every in-neighbour whose hops are non-critical
+    const bool sym = (ri.flags & 1u) != 0;
+    const uint32_t* icol = sym ? a.col + ri.e_off : a.icol + ri.ie_off;
+    const float* iwp = sym ? a.w + ri.e_off : a.iw + ri.ie_off;
+    const uint32_t ib = sym ? rb : a.irp[ri.irp_off + v];
+    const uint32_t ie = sym ? re : a.irp[ri.irp_off + v + 1];
+    for (uint32_t q = ib; q < ie; ++q) {
+        const uint32_t u = icol[q];
+        const double wq = weighted ? (double)iwp[q] : 1.0;
+        const int cu = n.C > 1 ? a.comp[ri.comp_off + u] : 0;
+#pragma unroll
+        for (int s = 0; s < MS; ++s) {
+            if (s >= n.S) break;
+            const uint32_t xu = X[ri.x_off + (uint64_t)s * N + u];
+            const double du = n.d[cu * n.S + s];
+            if (!hop_active(du, xu, nc)) continue;
+            const double ain = (du * (double)xu) * wq;
+            mu[s] += ain;
+            sig[s] += ain;
+        }
+    }
+    double tp = kInf;
+    const double eps = a.epsilon;
+#pragma unroll
+    for (int s = 0; s < MS; ++s) {
+        if (s >= n.S) break;
+        const int hor = n.hor[s];
+        if (hor == 0) continue;
+        const double gg = g_cao(hor, n.mult[s], x[s]);
+        const double mm = fmax((eps * (double)x[s]) / gg, 1.0);
+        const double t1 = mm / __builtin_fabs(mu[s]);
+        const double t2 = (mm * mm) / sig[s];
+        tp = fmin(tp, fmin(t1, t2));
+    }
+    tp_out = tp;
+    a0c_out = a0c;
+}
+
+__device__ __forceinline__ void reduce_partA(double& tp, double& t2, uint32_t& v2) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double otp = __shfl_xor(tp, off);
+        const double ot2 = __shfl_xor(t2, off);
+        const uint32_t ov = (uint32_t)__shfl_xor((int)v2, off);
+        tp = fmin(tp, otp);
+        if (ot2 < t2 || (ot2 == t2 && ov < v2)) {
+            t2 = ot2;
+            v2 = ov;
+        }
+    }
+}
+
+template <int MS, int MR>
+__device__ void finalize_A(const KernelArgs& a, const NetView& n, const RealInfo& ri, uint32_t r,
+                           double tp, double t2, uint32_t v2) {
+    RealCtl c = a.ctl[r];
+    if (tp == kInf && t2 == kInf) {
+        c.status = kAbsorbed;
+        c.done = 1u;
+        a.ctl[r] = c;
+        atomicAdd(a.ndone, 1u);
+        return;
+    }
+    c.tau_p = tp;
+    c.tau2 = t2;
+    c.vstar = v2;
+    c.attempt = 0u;
+    c.ev_kind = 0;
+    c.ev_idx = -1;
+    c.ev_dst = v2;
+    decide(c);
+    if (t2 < kInf) choose_event<MS, MR>(a, n, ri, c, c.cur ? a.x1 : a.x0);
+    a.ctl[r] = c;
+}
+
+template <int MS, int MR>
+__global__ __launch_bounds__(256) void phase_a_kernel(KernelArgs a) {
+    const NetView n = net_view(a);
+    const uint32_t lane = threadIdx.x & 63u;
+    for (uint32_t it = 0;; ++it) {
+        const uint32_t t = tile_at(it, a.ntiles, a.waves_per_block);
+        if (t == 0xFFFFFFFFu) break;
+        const TileEnt te = a.tiles[t];
+        const uint32_t r = wave_uniform(te.r);
+        const RealInfo ri = a.info[r];
+        const uint32_t done = wave_uniform(a.ctl[r].done), retry = wave_uniform(a.ctl[r].retry);
+        const bool dbg = a.dbg_tp != nullptr;
+        if (!dbg && (done || retry)) continue;
+        const uint32_t cur = wave_uniform(a.ctl[r].cur);
+        const uint32_t* X = cur ? a.x1 : a.x0;
+        const uint32_t v = te.v0 + lane;
+        double tp = kInf, t2 = kInf;
+        uint32_t v2 = v;
+        if (v < ri.N) {
+            double a0c;
+            tau_subvolume<MS, MR>(a, n, ri, X, v, tp, a0c);
+            if (a0c > 0.0) {
+                const uint32_t step = wave_uniform(a.ctl[r].step);
+                const U4 o = philox10(v, kSelfId, step, ctr_word3(ri.id, 0u, kTau2, 0u), a.seed_lo,
+                                      a.seed_hi);
+                t2 = -dlog(uniform01(o.x)) / a0c;
+            }
+            if (dbg) {
+                const uint64_t base = ri.x_off / (uint64_t)n.S;
+                a.dbg_tp[base + v] = tp;
+                a.dbg_a0c[base + v] = a0c;
+                a.dbg_t2[base + v] = t2;
+            }
+        }
+        if (dbg) continue;
+        reduce_partA(tp, t2, v2);
+        const uint32_t g = ri.group_begin + (t - ri.tile_begin) / kGroup;
+        const uint32_t gfirst = ri.tile_begin + (g - ri.group_begin) * kGroup;
+        const uint32_t ng = min((uint32_t)kGroup, ri.tile_begin + ri.ntiles - gfirst);
+        int last = 0;
+        if (lane == 0) {
+            a.partA_tile[t] = PartA{tp, t2, v2, 0u};
+            __threadfence();
+            last = atomicAdd(&a.cntA_grp[g], 1u) == ng - 1u;
+        }
+        last = __shfl(last, 0);
+        if (!last) continue;
+        __threadfence();
+        {
+            PartA p = lane < ng ? a.partA_tile[gfirst + lane] : PartA{kInf, kInf, 0xFFFFFFFFu, 0u};
+            tp = p.tp;
+            t2 = p.t2;
+            v2 = p.v2;
+        }
+        reduce_partA(tp, t2, v2);
+        int lastr = 0;
+        if (lane == 0) {
+            a.partA_grp[g] = PartA{tp, t2, v2, 0u};
+            a.cntA_grp[g] = 0u;
+            __threadfence();
+            lastr = atomicAdd(&a.cntA_real[r], 1u) == ri.ngroups - 1u;
+        }
+        lastr = __shfl(lastr, 0);
+        if (!lastr) continue;
+        __threadfence();
+        tp = kInf;
+        t2 = kInf;
+        v2 = 0xFFFFFFFFu;
+        for (uint32_t q = lane; q < ri.ngroups; q += 64u) {
+            const PartA p = a.partA_grp[ri.group_begin + q];
+            tp = fmin(tp, p.tp);
+            if (p.t2 < t2 || (p.t2 == t2 && p.v2 < v2)) {
+                t2 = p.t2;
+                v2 = p.v2;
+            }
+        }
+        reduce_partA(tp, t2, v2);
+        if (lane == 0) {
+            a.cntA_real[r] = 0u;
+            finalize_A<MS, MR>(a, n, ri, r, tp, t2, v2);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// phase B: leap + critical event + negative check, commit / leap-back
+// ---------------------------------------------------------------------------
+struct StepView {
+    double tau;
+    uint32_t step, attempt, real;
+    int crit, ev_kind, ev_idx;
+    uint32_t vstar, ev_dst;
+};
+
+template <int MS, int MR>
+__device__ __forceinline__ bool leap_subvolume(const KernelArgs& a, const NetView& n, const RealInfo& ri,
+                                               const uint32_t* X, uint32_t* Xn, uint32_t v,
+                                               const StepView& sv, unsigned long long& fire_out) {
+    const uint32_t N = ri.N;
+    const int nc = a.n_critical;
+    uint32_t x[MS];
+#pragma unroll
+    for (int s = 0; s < MS; ++s) x[s] = s < n.S ? X[ri.x_off + (uint64_t)s * N + v] : 0u;
+    const int c = n.C > 1 ? a.comp[ri.comp_off + v] : 0;
+    double av[MR];
+    uint32_t crit[(MR + 31) / 32];
+    react_props<MR, MS>(n, x, c, nc, av, crit);
+    long long nx[MS];
+#pragma unroll
+    for (int s = 0; s < MS; ++s) nx[s] = (long long)x[s];
+    unsigned long long fire = 0;
+    const double tau = sv.tau;
+    const uint32_t k0 = a.seed_lo, k1 = a.seed_hi;
+
+    // non-critical reactions: k_j ~ Poisson(tau a_j)
+    {
+        uint32_t blk = 0xFFFFFFFFu;
+        U4 o{0, 0, 0, 0};
+        const uint32_t c3base = ctr_word3(sv.real, sv.attempt, kReact, 0u);
+        for (int j = 0; j < n.R; ++j) {
+            if (is_crit<MR>(crit, j) || !(av[j] > 0.0)) continue;
+            const uint32_t b = (uint32_t)j >> 2;
+            if (b != blk) {
+                o = philox10(v, kSelfId, sv.step, c3base | b, k0, k1);
+                blk = b;
+            }
+            const uint32_t wd = (uint32_t)j & 3u;
+            const DrawId id{v, kSelfId, sv.step, c3base | b, k0, k1, wd};
+            const long long k = poisson(tau * av[j], pick(o, wd), id);
+            fire += (unsigned long long)k;
+#pragma unroll
+            for (int s = 0; s < MS; ++s)
+                if (s < n.S) nx[s] += k * (long long)n.nu[s * n.R + j];
+        }
+    }
+
+    const bool weighted = (ri.flags & 2u) != 0;
+    const bool sym = (ri.flags & 1u) != 0;
+    const uint32_t rb = a.rp[ri.rp_off + v], re = a.rp[ri.rp_off + v + 1];
+    double dx[MS];
+    bool act[MS];
+#pragma unroll
+    for (int s = 0; s < MS; ++s) {
+        const double d = s < n.S ? n.d[c * n.S + s] : 0.0;
+        act[s] = s < n.S && hop_active(d, x[s], nc);
+        dx[s] = d * (double)x[s];
+    }
+    const uint32_t c3hop = ctr_word3(sv.real, sv.attempt, kHop, 0u);
+    if (sym) {
+        // one Philox block per (edge, species pair) serves both directions
+        for (uint32_t q = rb; q < re; ++q) {
+            const uint32_t u = a.col[ri.e_off + q];
+            const double wq = weighted ? (double)a.w[ri.e_off + q] : 1.0;
+            const int cu = n.C > 1 ? a.comp[ri.comp_off + u] : 0;
+            const uint32_t lo = v < u ? v : u, hi = v < u ? u : v;
+            const uint32_t dout = v > u ? 1u : 0u, din = 1u - dout;
+#pragma unroll
+            for (int s0 = 0; s0 < MS; s0 += 2) {
+                if (s0 >= n.S) break;
+                double lout[2], lin[2];
+                bool any = false;
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int s = s0 + h;
+                    lout[h] = 0.0;
+                    lin[h] = 0.0;
+                    if (s >= MS || s >= n.S) continue;
+                    if (act[s]) lout[h] = tau * (dx[s] * wq);
+                    const uint32_t xu = X[ri.x_off + (uint64_t)s * N + u];
+                    const double du = n.d[cu * n.S + s];
+                    if (hop_active(du, xu, nc)) lin[h] = tau * ((du * (double)xu) * wq);
+                    any = any || lout[h] > 0.0 || lin[h] > 0.0;
+                }
+                if (!any) continue;
+                const uint32_t b = (uint32_t)s0 >> 1;
+                const U4 o = philox10(lo, hi, sv.step, c3hop | b, k0, k1);
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int s = s0 + h;
+                    if (s >= MS || s >= n.S) continue;
+                    const uint32_t wo = 2u * (uint32_t)h + dout, wi = 2u * (uint32_t)h + din;
+                    if (lout[h] > 0.0) {
+                        const DrawId id{lo, hi, sv.step, c3hop | b, k0, k1, wo};
+                        const long long k = poisson(lout[h], pick(o, wo), id);
+                        nx[s] -= k;
+                        fire += (unsigned long long)k;
+                    }
+                    if (lin[h] > 0.0) {
+                        const DrawId id{lo, hi, sv.step, c3hop | b, k0, k1, wi};
+                        nx[s] += poisson(lin[h], pick(o, wi), id);
+                    }
+                }
+            }
+        }
+    } else {
+        for (uint32_t q = rb; q < re; ++q) {
+            const uint32_t u = a.col[ri.e_off + q];
+            const double wq = weighted ? (double)a.w[ri.e_off + q] : 1.0;
+            const uint32_t lo = v < u ? v : u, hi = v < u ? u : v;
+            const uint32_t dout = v > u ? 1u : 0u;
+#pragma unroll
+            for (int s = 0; s < MS; ++s) {
+                if (s >= n.S) break;
+                if (!act[s]) continue;
+                const uint32_t wd = 2u * (uint32_t)s + dout;
+                const U4 o = philox10(lo, hi, sv.step, c3hop | (wd >> 2), k0, k1);
+                const DrawId id{lo, hi, sv.step, c3hop | (wd >> 2), k0, k1, wd & 3u};
+                const long long k = poisson(tau * (dx[s] * wq), pick(o, wd & 3u), id);
+                nx[s] -= k;
+                fire += (unsigned long long)k;
+            }
+        }
+        const uint32_t ib = a.irp[ri.irp_off + v], ie = a.irp[ri.irp_off + v + 1];
+        for (uint32_t q = ib; q < ie; ++q) {
+            const uint32_t u = a.icol[ri.ie_off + q];
+            const double wq = weighted ? (double)a.iw[ri.ie_off + q] : 1.0;
+            const int cu = n.C > 1 ? a.comp[ri.comp_off + u] : 0;
+            const uint32_t lo = v < u ? v : u, hi = v < u ? u : v;
+            const uint32_t din = u > v ? 1u : 0u;
+#pragma unroll
+            for (int s = 0; s < MS; ++s) {
+                if (s >= n.S) break;
+                const uint32_t xu = X[ri.x_off + (uint64_t)s * N + u];
+                const double du = n.d[cu * n.S + s];
+                if (!hop_active(du, xu, nc)) continue;
+                const uint32_t wd = 2u * (uint32_t)s + din;
+                const U4 o = philox10(lo, hi, sv.step, c3hop | (wd >> 2), k0, k1);
+                const DrawId id{lo, hi, sv.step, c3hop | (wd >> 2), k0, k1, wd & 3u};
+                nx[s] += poisson(tau * ((du * (double)xu) * wq), pick(o, wd & 3u), id);
+            }
+        }
+    }
+
+    // the critical event (one per global step, in the argmin-tau'' subvolume)
+    if (sv.crit) {
+        if (v == sv.vstar) {
+            if (sv.ev_kind == 1) {
+#pragma unroll
+                for (int s = 0; s < MS; ++s)
+                    if (s < n.S) nx[s] += n.nu[s * n.R + sv.ev_idx];
+                fire += 1ull;
+            } else if (sv.ev_kind == 2) {
+#pragma unroll
+                for (int s = 0; s < MS; ++s)
+                    if (s == sv.ev_idx) nx[s] -= 1;
+                fire += 1ull;
+            }
+        }
+        if (sv.ev_kind == 2 && v == sv.ev_dst && sv.ev_dst != sv.vstar) {
+#pragma unroll
+            for (int s = 0; s < MS; ++s)
+                if (s == sv.ev_idx) nx[s] += 1;
+        }
+    }
+    bool bad = false;
+#pragma unroll
+    for (int s = 0; s < MS; ++s) {
+        if (s >= n.S) break;
+        bad = bad || nx[s] < 0 || nx[s] > 0xFFFFFFFFll;
+        Xn[ri.x_off + (uint64_t)s * N + v] = (uint32_t)nx[s];
+    }
+    fire_out = fire;
+    return bad;
+}
+
+__device__ __forceinline__ void reduce_partB(unsigned long long& fire, uint32_t& bad) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        fire += __shfl_xor(fire, off);
+        bad |= (uint32_t)__shfl_xor((int)bad, off);
+    }
+}
+
+__device__ void finalize_B(const KernelArgs& a, const RealInfo& ri, uint32_t r, unsigned long long fire,
+                           uint32_t bad) {
+    RealCtl c = a.ctl[r];
+    if (bad) {
+        c.leapbacks += 1;
+        c.attempt += 1u;
+        if (c.attempt > kMaxAttempt) {
+            c.status = kErrLeapback;
+            c.done = 1u;
+            a.ctl[r] = c;
+            atomicAdd(a.ndone, 1u);
+            return;
+        }
+        c.tau_p = c.tau_p * 0.5;
+        decide(c);
+        c.retry = 1u;
+        a.ctl[r] = c;
+        return;
+    }
+    c.cur ^= 1u;
+    c.t = c.last ? c.t_end : c.t + c.tau;
+    c.step += 1u;
+    c.steps += 1;
+    c.firings += fire;
+    c.subvolume_leaps += ri.N;
+    if (c.crit) c.crit_events += 1;
+    c.attempt = 0u;
+    c.retry = 0u;
+    if (c.t >= c.t_end || c.steps >= c.step_limit || c.t >= c.t_pause) {
+        c.done = 1u;
+        if (c.t >= c.t_end || c.steps >= c.step_limit) c.status = kDone;
+        atomicAdd(a.ndone, 1u);
+    }
+    a.ctl[r] = c;
+}
+
+template <int MS, int MR>
+__global__ __launch_bounds__(256) void phase_b_kernel(KernelArgs a) {
+    const NetView n = net_view(a);
+    const uint32_t lane = threadIdx.x & 63u;
+    for (uint32_t it = 0;; ++it) {
+        const uint32_t t = tile_at(it, a.ntiles, a.waves_per_block);
+        if (t == 0xFFFFFFFFu) break;
+        const TileEnt te = a.tiles[t];
+        const uint32_t r = wave_uniform(te.r);
+        const RealInfo ri = a.info[r];
+        const RealCtl* cp = &a.ctl[r];
+        if (wave_uniform(cp->done)) continue;
+        StepView sv;
+        sv.tau = cp->tau;
+        sv.step = wave_uniform(cp->step);
+        sv.attempt = wave_uniform(cp->attempt);
+        sv.real = ri.id;
+        sv.crit = (int)wave_uniform((uint32_t)cp->crit);
+        sv.ev_kind = (int)wave_uniform((uint32_t)cp->ev_kind);
+        sv.ev_idx = (int)wave_uniform((uint32_t)cp->ev_idx);
+        sv.vstar = wave_uniform(cp->vstar);
+        sv.ev_dst = wave_uniform(cp->ev_dst);
+        const uint32_t cur = wave_uniform(cp->cur);
+        const uint32_t* X = cur ? a.x1 : a.x0;
+        uint32_t* Xn = cur ? a.x0 : a.x1;
+        const uint32_t v = te.v0 + lane;
+        unsigned long long fire = 0;
+        uint32_t bad = 0;
+        if (v < ri.N) bad = leap_subvolume<MS, MR>(a, n, ri, X, Xn, v, sv, fire) ? 1u : 0u;
+        reduce_partB(fire, bad);
+        const uint32_t g = ri.group_begin + (t - ri.tile_begin) / kGroup;
+        const uint32_t gfirst = ri.tile_begin + (g - ri.group_begin) * kGroup;
+        const uint32_t ng = min((uint32_t)kGroup, ri.tile_begin + ri.ntiles - gfirst);
+        int last = 0;
+        if (lane == 0) {
+            a.partB_tile[t] = PartB{fire, bad, 0u};
+            __threadfence();
+            last = atomicAdd(&a.cntB_grp[g], 1u) == ng - 1u;
+        }
+        last = __shfl(last, 0);
+        if (!last) continue;
+        __threadfence();
+        {
+            const PartB p = lane < ng ? a.partB_tile[gfirst + lane] : PartB{0ull, 0u, 0u};
+            fire = p.fire;
+            bad = p.bad;
+        }
+        reduce_partB(fire, bad);
+        int lastr = 0;
+        if (lane == 0) {
+            a.partB_grp[g] = PartB{fire, bad, 0u};
+            a.cntB_grp[g] = 0u;
+            __threadfence();
+            lastr = atomicAdd(&a.cntB_real[r], 1u) == ri.ngroups - 1u;
+        }
+        lastr = __shfl(lastr, 0);
+        if (!lastr) continue;
+        __threadfence();
+        fire = 0;
+        bad = 0;
+        for (uint32_t q = lane; q < ri.ngroups; q += 64u) {
+            const PartB p = a.partB_grp[ri.group_begin + q];
+            fire += p.fire;
+            bad |= p.bad;
+        }
+        reduce_partB(fire, bad);
+        if (lane == 0) {
+            a.cntB_real[r] = 0u;
+            finalize_B(a, ri, r, fire, bad);
+        }
+    }
+}
+
+// Explicit instantiations: Schlogl-like (1 species, <= 4 reactions), small
+// networks (<= 8 species, <= 16 reactions), and the general case.
+#define PLIS_INST(MS, MR)                                                   \
+    template __global__ void phase_a_kernel<MS, MR>(KernelArgs);            \
+    template __global__ void phase_b_kernel<MS, MR>(KernelArgs);
+PLIS_INST(1, 4)
+PLIS_INST(8, 16)
+PLIS_INST(64, 128)
+
+}  // namespace plis

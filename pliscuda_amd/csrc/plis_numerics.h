@@ -1,0 +1,201 @@
+// plis_numerics.h -- gfx950 device numerics for the RDME tau-leap step.
+//
+// Counter-based Philox4x32-10 (Salmon et al., SC'11), uniform doubles from
+// 32-bit words, exp / log / sqrt / log k! written from published methods with
+// every fused multiply-add explicit (the library is built with
+// -ffp-contract=off), and a Poisson sampler: sequential-search inversion for
+// mean < 10 (Devroye 1986, ch. X.3) and PTRS transformed rejection for
+// mean >= 10 (Hoermann 1993).  The host CPU restatement applies the same
+// operations in the same order, so device and host draws agree bit for bit.
+//
+// These replace the curand calls of the reference (curand_init
+// PLISCuda/src/init.cu:3-13, curand_poisson leap.cu:27,51, curand_uniform
+// tau.cu:193, leap.cu:89,182, ssa.cu:64,90).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace plis {
+
+// ---- constants (60-digit decimal evaluation, rounded to double) ----
+constexpr double kLn2Hi = 0x1.62e42fefa3800p-1;  // ln2, 42 significant bits
+constexpr double kLn2Lo = 0x1.ef35793c76730p-45;
+constexpr double kInvLn2 = 0x1.71547652b82fep+0;
+constexpr double kSqrt2 = 0x1.6a09e667f3bcdp+0;
+constexpr double kHalfLog2Pi = 0x1.d67f1c864beb5p-1;
+constexpr double kC12 = 0x1.5555555555555p-4;    // 1/12
+constexpr double kC360 = 0x1.6c16c16c16c17p-9;   // 1/360
+constexpr double kC1260 = 0x1.a01a01a01a01ap-11; // 1/1260
+constexpr double kPoissonInvMax = 10.0;
+
+// Philox stream purposes (counter word 3, bits 7..9).
+enum Purpose : uint32_t { kTau2 = 1, kCrit = 2, kReact = 3, kHop = 4, kExtra = 5 };
+constexpr uint32_t kSelfId = 0xFFFFFFFFu;
+constexpr uint32_t kMaxAttempt = 63u;
+
+__device__ __forceinline__ uint32_t ctr_word3(uint32_t real, uint32_t attempt, uint32_t purpose,
+                                              uint32_t block) {
+    return (real << 16) | (attempt << 10) | (purpose << 7) | block;
+}
+
+struct U4 { uint32_t x, y, z, w; };
+
+__device__ __forceinline__ U4 philox10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                       uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int i = 0; i < 10; ++i) {
+        const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+        const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+        c0 = hi1 ^ c1 ^ k0;
+        c2 = hi0 ^ c3 ^ k1;
+        c1 = lo1;
+        c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return U4{c0, c1, c2, c3};
+}
+
+__device__ __forceinline__ uint32_t pick(const U4& o, uint32_t i) {
+    return i == 0 ? o.x : (i == 1 ? o.y : (i == 2 ? o.z : o.w));
+}
+
+__device__ __forceinline__ double uniform01(uint32_t w) { return ((double)w + 0.5) * 0x1p-32; }
+
+// exp(x): Cody-Waite reduction r = x - n ln2 (two fma steps), degree-13
+// Taylor polynomial of e^r in Horner form, scale by 2^n built from bits.
+__device__ __forceinline__ double dexp(double x) {
+    if (x < -708.0) return 0.0;
+    if (x > 709.0) return __builtin_huge_val();
+    const double n = __builtin_rint(x * kInvLn2);
+    double r = __builtin_fma(-n, kLn2Hi, x);
+    r = __builtin_fma(-n, kLn2Lo, r);
+    double p = 0x1.6124613a86d09p-33;            // 1/13!
+    p = __builtin_fma(p, r, 0x1.1eed8eff8d898p-29);  // 1/12!
+    p = __builtin_fma(p, r, 0x1.ae64567f544e4p-26);  // 1/11!
+    p = __builtin_fma(p, r, 0x1.27e4fb7789f5cp-22);  // 1/10!
+    p = __builtin_fma(p, r, 0x1.71de3a556c734p-19);  // 1/9!
+    p = __builtin_fma(p, r, 0x1.a01a01a01a01ap-16);  // 1/8!
+    p = __builtin_fma(p, r, 0x1.a01a01a01a01ap-13);  // 1/7!
+    p = __builtin_fma(p, r, 0x1.6c16c16c16c17p-10);  // 1/6!
+    p = __builtin_fma(p, r, 0x1.1111111111111p-7);   // 1/5!
+    p = __builtin_fma(p, r, 0x1.5555555555555p-5);   // 1/4!
+    p = __builtin_fma(p, r, 0x1.5555555555555p-3);   // 1/3!
+    p = __builtin_fma(p, r, 0x1.0000000000000p-1);   // 1/2!
+    p = __builtin_fma(p, r, 1.0);
+    p = __builtin_fma(p, r, 1.0);
+    const long long ni = (long long)n;
+    return p * __longlong_as_double((ni + 1023) << 52);
+}
+
+// log(x) for normal x > 0: x = m 2^e with m in (sqrt2/2, sqrt2],
+// log m = 2 atanh(s), s = (m-1)/(m+1), odd series through s^21.
+__device__ __forceinline__ double dlog(double x) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+    int e = (int)((b >> 52) & 0x7FF) - 1023;
+    double m = __longlong_as_double((long long)((b & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull));
+    if (m > kSqrt2) { m = m * 0.5; e += 1; }
+    const double s = (m - 1.0) / (m + 1.0);
+    const double z = s * s;
+    double P = 0x1.8618618618618p-5;                 // 1/21
+    P = __builtin_fma(P, z, 0x1.af286bca1af28p-5);   // 1/19
+    P = __builtin_fma(P, z, 0x1.e1e1e1e1e1e1ep-5);   // 1/17
+    P = __builtin_fma(P, z, 0x1.1111111111111p-4);   // 1/15
+    P = __builtin_fma(P, z, 0x1.3b13b13b13b14p-4);   // 1/13
+    P = __builtin_fma(P, z, 0x1.745d1745d1746p-4);   // 1/11
+    P = __builtin_fma(P, z, 0x1.c71c71c71c71cp-4);   // 1/9
+    P = __builtin_fma(P, z, 0x1.2492492492492p-3);   // 1/7
+    P = __builtin_fma(P, z, 0x1.999999999999ap-3);   // 1/5
+    P = __builtin_fma(P, z, 0x1.5555555555555p-2);   // 1/3
+    P = __builtin_fma(P, z, 1.0);
+    const double t = (2.0 * s) * P;
+    const double ed = (double)e;
+    return __builtin_fma(ed, kLn2Hi, __builtin_fma(ed, kLn2Lo, t));
+}
+
+// sqrt by six Newton-Heron steps from an exponent-halving first guess.
+__device__ __forceinline__ double dsqrt(double x) {
+    if (!(x > 0.0)) return 0.0;
+    double y = __longlong_as_double(
+        (long long)(((unsigned long long)__double_as_longlong(x) >> 1) + (0x3FF0000000000000ull >> 1)));
+#pragma unroll
+    for (int i = 0; i < 6; ++i) y = 0.5 * (y + x / y);
+    return y;
+}
+
+// log k!: exact table below 10, Stirling series through 1/z^5 above.
+__device__ __forceinline__ double dlogfact(long long k) {
+    if (k < 10) {
+        switch ((int)k) {
+            case 2: return 0x1.62e42fefa39efp-1;
+            case 3: return 0x1.cab0bfa2a2002p+0;
+            case 4: return 0x1.96ca77c922cf9p+1;
+            case 5: return 0x1.326643c4479c9p+2;
+            case 6: return 0x1.a51273acf01cap+2;
+            case 7: return 0x1.10ce1f32dcc30p+3;
+            case 8: return 0x1.5358e82fcb70dp+3;
+            case 9: return 0x1.99a8921a7f7cfp+3;
+            default: return 0.0;
+        }
+    }
+    const double z = (double)k + 1.0;
+    const double z2 = z * z;
+    return (((double)k + 0.5) * dlog(z) - z) + kHalfLog2Pi + (kC12 - (kC360 - kC1260 / z2) / z2) / z;
+}
+
+// Draw identity: main counter (c0..c3 with purpose/block), key, word in block.
+struct DrawId {
+    uint32_t c0, c1, c2, c3, k0, k1, word;
+};
+
+// q-th uniform of the draw's PTRS stream: purpose replaced by kExtra, key0
+// perturbed by an odd multiplier of (4*call + word + 1).
+__device__ __forceinline__ double extra_uniform(const DrawId& id, uint32_t q) {
+    const uint32_t c3 = (id.c3 & ~(7u << 7)) | (kExtra << 7);
+    const uint32_t k0 = id.k0 ^ (0x85EBCA6Bu * (4u * (q >> 2) + id.word + 1u));
+    const U4 o = philox10(id.c0, id.c1, id.c2, c3, k0, id.k1);
+    return uniform01(pick(o, q & 3u));
+}
+
+__device__ __attribute__((noinline)) long long poisson_ptrs(double lam, DrawId id) {
+    const double slam = dsqrt(lam);
+    const double loglam = dlog(lam);
+    const double b = 0.931 + 2.53 * slam;
+    const double a = -0.059 + 0.02483 * b;
+    const double invalpha = 1.1239 + 1.1328 / (b - 3.4);
+    const double vr = 0.9277 - 3.6224 / (b - 2.0);
+    const double loginvalpha = dlog(invalpha);
+    for (uint32_t it = 0; it < 100000u; ++it) {
+        const double U = extra_uniform(id, 2u * it) - 0.5;
+        const double V = extra_uniform(id, 2u * it + 1u);
+        const double us = 0.5 - __builtin_fabs(U);
+        const double kd = __builtin_floor((2.0 * a / us + b) * U + lam + 0.43);
+        if (us >= 0.07 && V <= vr) return (long long)kd;
+        if (kd < 0.0 || (us < 0.013 && V > us)) continue;
+        const long long k = (long long)kd;
+        const double lhs = (dlog(V) + loginvalpha) - dlog(a / (us * us) + b);
+        const double rhs = (-lam + kd * loglam) - dlogfact(k);
+        if (lhs <= rhs) return k;
+    }
+    return (long long)__builtin_floor(lam);
+}
+
+// Poisson(lam) from the packed word u0 (inversion) or the draw's PTRS stream.
+__device__ __forceinline__ long long poisson(double lam, uint32_t u0, const DrawId& id) {
+    if (!(lam > 0.0)) return 0;
+    if (lam < kPoissonInvMax) {
+        const double u = uniform01(u0);
+        double p = dexp(-lam);
+        double F = p;
+        long long k = 0;
+        while (u > F && k < 1000) {
+            k += 1;
+            p = (p * lam) / (double)k;
+            F = F + p;
+        }
+        return k;
+    }
+    return poisson_ptrs(lam, id);
+}
+
+}  // namespace plis

@@ -1,0 +1,116 @@
+// plis_types.h -- device-resident layout of a batch of RDME realizations.
+//
+// HBM layout (one context = one batch of realizations sharing a reaction network):
+//   counts  : two ping-pong uint32 buffers; realization r occupies
+//             [x_off, x_off + S*N) species-major ([s*N + v], coalesced per
+//             species like the reference's GET_SPI, PLISCuda/include/cuda/constants.cuh:27)
+//   topology: canonical CSR per realization (row_ptr relative to e_off, col
+//             local ids, optional float weights); a separate in-CSR only
+//             when the graph is not symmetric
+//   tiles   : 64 consecutive subvolumes of one realization = one wavefront;
+//             64 tiles = one reduction group
+//   control : one RealCtl per realization (step, clocks, tau, event, stats)
+#pragma once
+#include <stdint.h>
+
+namespace plis {
+
+constexpr int kTile = 64;        // subvolumes per tile (one wavefront)
+constexpr int kGroup = 64;       // tiles per reduction group
+constexpr int kMaxS = 64;
+constexpr int kMaxR = 128;
+
+struct NetHeader {
+    int32_t S, R, C, pad;
+    // offsets (in units of their element type) into the network blob
+    int32_t off_nu;      // int32 [s*R + j]
+    int32_t off_rs;      // int32 [j*3 + i] reactant species
+    int32_t off_rr;      // int32 [j*3 + i] reactant multiplicity
+    int32_t off_nrs;     // int32 [j]
+    int32_t off_order;   // int32 [j]
+    int32_t off_hor;     // int32 [s]
+    int32_t off_mult;    // int32 [s]
+    int32_t off_k;       // double [c*R + j] (offset in doubles)
+    int32_t off_d;       // double [c*S + s] (offset in doubles)
+    int32_t ints_total, dbl_total;
+};
+
+struct RealInfo {
+    uint64_t x_off;      // counts offset (elements)
+    uint64_t e_off;      // out-CSR edge base
+    uint64_t ie_off;     // in-CSR edge base (== e_off when symmetric)
+    uint64_t rp_off;     // out row_ptr base (N+1 entries)
+    uint64_t irp_off;    // in row_ptr base
+    uint64_t comp_off;   // compartment ids base (if C > 1)
+    uint32_t N, id;
+    uint32_t tile_begin, ntiles;
+    uint32_t group_begin, ngroups;
+    uint32_t flags;      // bit0: symmetric (in-CSR aliases out-CSR), bit1: weighted
+    uint32_t pad;
+};
+
+enum CtlStatus : int32_t { kRunning = 0, kDone = 1, kAbsorbed = 2, kErrLeapback = -3, kErrOverflow = -4 };
+
+struct RealCtl {
+    double t, t_end, t_pause;
+    double tau_p, tau2, tau;
+    uint64_t step_limit;
+    uint64_t steps, leapbacks, firings, subvolume_leaps, crit_events;
+    uint32_t step, attempt, vstar, ev_dst;
+    int32_t crit, last, ev_kind, ev_idx;
+    uint32_t done, retry, cur;
+    int32_t status;
+};
+
+struct TileEnt {
+    uint32_t r, v0;
+};
+
+// Per-step partials.
+struct PartA {
+    double tp, t2;
+    uint32_t v2, pad;
+};
+struct PartB {
+    unsigned long long fire;
+    uint32_t bad, pad;
+};
+
+struct KernelArgs {
+    const int32_t* net_i;
+    const double* net_d;
+    NetHeader nh;
+    const RealInfo* info;
+    RealCtl* ctl;
+    const TileEnt* tiles;
+    uint32_t ntiles;
+    uint32_t nreal;
+    uint32_t* x0;
+    uint32_t* x1;
+    const uint32_t* rp;
+    const uint32_t* col;
+    const float* w;
+    const uint32_t* irp;
+    const uint32_t* icol;
+    const float* iw;
+    const int32_t* comp;
+    PartA* partA_tile;
+    PartA* partA_grp;
+    PartB* partB_tile;
+    PartB* partB_grp;
+    uint32_t* cntA_grp;
+    uint32_t* cntA_real;
+    uint32_t* cntB_grp;
+    uint32_t* cntB_real;
+    uint32_t* ndone;
+    double epsilon;
+    int32_t n_critical;
+    uint32_t seed_lo, seed_hi;
+    // debug outputs of phase A (NULL in normal runs): per-subvolume tau', a0c, tau2
+    double* dbg_tp;
+    double* dbg_a0c;
+    double* dbg_t2;
+    uint32_t waves_per_block;
+};
+
+}  // namespace plis
