@@ -1,0 +1,214 @@
+#!/usr/bin/env python3
+"""Benchmark: subvolume-leaps/s and firings/s of the RDME tau-leap step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload brusselator|schlogl|batch]
+
+A "step" is one iteration of the device-resident step loop over the whole
+workload: one tau pass (propensities, Cao tau', tau'', global min) and one
+leap pass (Poisson firings, gather-form diffusion, critical event, negativity
+check, commit or leap-back).  The headline workload is the 1,048,576-
+subvolume Brusselator random mesh (128x128x64, degree 4..12, weighted edges,
+5 species, 8 reactions) with plain Cao tau; inputs are resident in HBM
+before timing starts.  value = accepted subvolume-leaps per second
+(a subvolume advanced by one accepted global step).
+
+The CPU baseline is the oracle (same algorithm and Philox streams) on the
+host cores over a bounded sample of the same workload; its final state is
+compared with the GPU's after the same number of steps (full-size parity).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+PMC_TRAFFIC = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+
+
+def algorithmic_bytes(S, topos, phase):
+    """Compulsory HBM bytes of one launch of a phase kernel (the counts,
+    CSR row pointers, neighbour ids and weights it must touch once)."""
+    tot = 0
+    for t in topos:
+        N = int(t["N"])
+        E = int(len(t["col"]))
+        topo = 4 * (N + 1) + 4 * E + (4 * E if t.get("w") is not None else 0)
+        counts = 4 * S * N * (2 if phase == "leap" else 1)
+        tot += counts + topo
+    return tot
+
+
+def build_workload(name):
+    from pliscuda_amd import workloads as W
+    if name == "brusselator":
+        return W.brusselator_mesh()
+    if name == "schlogl":
+        return W.schlogl()
+    if name == "batch":
+        return W.schlogl_batch()
+    raise SystemExit(f"unknown workload {name}")
+
+
+def gpu_measure(wl, steps, warmup, seed=2024, autotune=True):
+    import torch
+
+    from pliscuda_amd import capi
+    ctx = capi.Context(wl["net"], wl["topos"], wl["x0"], eps=wl["eps"], nc=wl["nc"], seed=seed)
+    t_end = wl["t_end"] if np.isfinite(wl["t_end"]) else float("inf")
+    ctx.arm(t_end=float("inf"))
+    if autotune:
+        ctx.autotune(3)
+    shape = ctx.launch_shape()
+    ctx.iterate(warmup)
+    ctx.sync()
+    s0 = [ctx.stats(r) for r in range(ctx.n)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ctx.iterate(steps)
+    ctx.sync()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    s1 = [ctx.stats(r) for r in range(ctx.n)]
+    acc = sum(b["steps"] - a["steps"] for a, b in zip(s0, s1))
+    leaps = sum(b["subvolume_leaps"] - a["subvolume_leaps"] for a, b in zip(s0, s1))
+    fir = sum(b["firings"] - a["firings"] for a, b in zip(s0, s1))
+    lbs = sum(b["leapbacks"] - a["leapbacks"] for a, b in zip(s0, s1))
+    simt = sum((b["t"] - a["t"]) * n for a, b, n in zip(s0, s1, ctx.N))
+    # per-kernel durations: HIP events around every launch on the context's stream
+    prof = ctx.profile(min(steps, 50))
+    n_prof = min(steps, 50)
+    ctx.close()
+    return {"seconds": dt, "accepted_steps": acc, "subvolume_leaps": leaps, "firings": fir,
+            "leapbacks": lbs, "subvolume_simtime": simt, "shape": shape,
+            "tau_ms": prof[1] / n_prof, "decide_ms": prof[2] / n_prof, "leap_ms": prof[3] / n_prof,
+            "commit_ms": prof[4] / n_prof, "iter_ms_events": prof[0] / n_prof}
+
+
+def cpu_baseline(wl, seed, sample_steps, gpu_check=True):
+    """Oracle on the host cores for sample_steps accepted steps (bounded)."""
+    from oracle import ffi
+    threads = ffi.num_threads()
+    t0 = time.perf_counter()
+    states, leaps, fires = [], 0, 0
+    nreal = len(wl["x0"])
+    use = min(nreal, 4)
+    for r in range(use):
+        m = ffi.Model(wl["net"], wl["topos"][r])
+        x = np.array(wl["x0"][r], dtype=np.uint32).copy()
+        st = ffi.run(m, x, wl["eps"], wl["nc"], seed, float("inf"), step_limit=sample_steps,
+                     realization=r)
+        states.append(x)
+        leaps += st.subvolume_leaps
+        fires += st.firings
+    dt = time.perf_counter() - t0
+    parity = None
+    if gpu_check:
+        from pliscuda_amd import capi
+        ctx = capi.Context(wl["net"], wl["topos"][:use], wl["x0"][:use], eps=wl["eps"], nc=wl["nc"],
+                           seed=seed)
+        ctx.run(float("inf"), step_limit=sample_steps)
+        parity = all(np.array_equal(ctx.get_state(r), states[r]) for r in range(use))
+        ctx.close()
+    return {"value": leaps / dt, "unit": "subvolume-leaps/s", "cores": threads, "kind": "port",
+            "firings_per_s": fires / dt,
+            "sample": f"{use} realization(s) x {sample_steps} accepted steps of the same workload "
+                      f"from the same initial state and seed ({dt:.1f} s of CPU work)",
+            "seconds": dt, "gpu_state_bit_identical": parity}
+
+
+def load_traffic(kernel):
+    try:
+        with open(PMC_TRAFFIC) as f:
+            d = json.load(f)
+        return d.get(kernel)
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="brusselator")
+    ap.add_argument("--cpu-sample-steps", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--extra", action="store_true", help="also measure the Schlogl grid and batch")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(realizations shard across GPUs by construction, see the design notes)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    wl = build_workload(args.workload)
+    S = int(wl["net"]["S"])
+    res = gpu_measure(wl, args.steps, args.warmup, autotune=not args.no_autotune)
+    value = res["subvolume_leaps"] / res["seconds"]
+    bytes_leap = algorithmic_bytes(S, wl["topos"], "leap")
+    bytes_tau = algorithmic_bytes(S, wl["topos"], "tau")
+    dominant = "leap" if res["leap_ms"] >= res["tau_ms"] else "tau"
+    dom_ms = res[f"{dominant}_ms"]
+    dom_bytes = bytes_leap if dominant == "leap" else bytes_tau
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    kname = "phase_b_kernel" if dominant == "leap" else "phase_a_kernel"
+    traffic = load_traffic(f"{args.workload}:{kname}")
+    out = {
+        "metric": "subvolume-leaps/s (plain Cao tau), 1 MI355X",
+        "value": value,
+        "unit": "subvolume-leaps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": res["seconds"] * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": wl["name"], "subvolumes": int(sum(t["N"] for t in wl["topos"])),
+                   "directed_edges": int(sum(len(t["col"]) for t in wl["topos"])),
+                   "species": S, "reactions": int(wl["net"]["R"]), "realizations": len(wl["topos"]),
+                   "epsilon": wl["eps"], "n_critical": wl["nc"], "tau": "plain Cao",
+                   "launch": res["shape"]},
+        "firings_per_s": res["firings"] / res["seconds"],
+        "accepted_steps": res["accepted_steps"],
+        "leapbacks": res["leapbacks"],
+        "subvolume_simtime_per_s": res["subvolume_simtime"] / res["seconds"],
+        "kernels": {"tau_ms_avg": res["tau_ms"], "leap_ms_avg": res["leap_ms"],
+                    "decide_ms_avg": res["decide_ms"], "commit_ms_avg": res["commit_ms"],
+                    "iteration_ms_events": res["iter_ms_events"],
+                    "tau_alg_bytes": bytes_tau, "leap_alg_bytes": bytes_leap,
+                    "tau_GBs": bytes_tau / (res["tau_ms"] * 1e-3) / 1e9,
+                    "leap_GBs": bytes_leap / (res["leap_ms"] * 1e-3) / 1e9},
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(wl, 2024, args.cpu_sample_steps)
+    if args.extra:
+        extra = {}
+        for name in ("schlogl", "batch"):
+            w2 = build_workload(name)
+            r2 = gpu_measure(w2, 200, 20)
+            extra[w2["name"]] = {"subvolume_leaps_per_s": r2["subvolume_leaps"] / r2["seconds"],
+                                 "firings_per_s": r2["firings"] / r2["seconds"],
+                                 "ms_per_step": r2["seconds"] * 1e3 / 200,
+                                 "tau_ms_avg": r2["tau_ms"], "leap_ms_avg": r2["leap_ms"],
+                                 "leap_GBs": algorithmic_bytes(1, w2["topos"], "leap") / (r2["leap_ms"] * 1e-3) / 1e9}
+        out["extra"] = extra
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

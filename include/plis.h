@@ -120,8 +120,8 @@ int plis_arm(plis_ctx *ctx, double t_end, double t_pause, uint64_t step_limit);
 int plis_iterate(plis_ctx *ctx, uint32_t n);
 
 /* Time n iterations with HIP events recorded on the context's stream around
- * every kernel launch; out[0] = total ms, out[1] = tau-kernel ms summed,
- * out[2] = leap-kernel ms summed. */
+ * every kernel launch; out[0] = total ms, then the summed ms of the tau pass,
+ * the decide kernel, the leap pass and the commit kernel (out[1..4]). */
 int plis_profile(plis_ctx *ctx, uint32_t n, double *out);
 
 int plis_set_state(plis_ctx *ctx, int32_t r, const uint32_t *x);

@@ -172,7 +172,8 @@ class Context:
         self._chk(lib().plis_iterate(self.h, n), "plis_iterate")
 
     def profile(self, n):
-        out = (C.c_double * 3)()
+        """[total, tau, decide, leap, commit] milliseconds summed over n iterations."""
+        out = (C.c_double * 5)()
         self._chk(lib().plis_profile(self.h, n, out), "plis_profile")
         return list(out)
 

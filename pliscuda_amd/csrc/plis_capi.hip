@@ -28,7 +28,7 @@ struct plis_ctx {
     int cus = 0;
     std::string err;
     NetHeader nh{};
-    int variant = 0;  // 0: <1,4>, 1: <8,16>, 2: <64,128>
+    int variant = 0;  // index into kVariantS / kVariantR
     int32_t S = 0, R = 0, C = 1;
     uint32_t nreal = 0, ntiles = 0, ngroups = 0;
     uint64_t total_counts = 0, total_sub = 0;
@@ -265,16 +265,35 @@ static KernelArgs make_args(plis_ctx* c) {
     return a;
 }
 
-static void launch_phase(plis_ctx* c, int phase, const KernelArgs& a) {
-    const dim3 grid(c->blocks), block(64 * c->wpb);
-    switch (c->variant * 2 + phase) {
-        case 0: hipLaunchKernelGGL((phase_a_kernel<1, 4>), grid, block, 0, c->stream, a); break;
-        case 1: hipLaunchKernelGGL((phase_b_kernel<1, 4>), grid, block, 0, c->stream, a); break;
-        case 2: hipLaunchKernelGGL((phase_a_kernel<8, 16>), grid, block, 0, c->stream, a); break;
-        case 3: hipLaunchKernelGGL((phase_b_kernel<8, 16>), grid, block, 0, c->stream, a); break;
-        case 4: hipLaunchKernelGGL((phase_a_kernel<64, 128>), grid, block, 0, c->stream, a); break;
-        default: hipLaunchKernelGGL((phase_b_kernel<64, 128>), grid, block, 0, c->stream, a); break;
+static const int kVariantS[] = {1, 3, 5, 8, 64};
+static const int kVariantR[] = {4, 4, 8, 16, 128};
+
+// Kernels of one step: 0 tau pass, 1 decide, 2 leap pass, 3 commit.
+static void launch_kernel(plis_ctx* c, int which, const KernelArgs& a) {
+    const dim3 grid(c->blocks), block(64 * c->wpb), rgrid(c->nreal), rblock(256);
+    if (which == 3) {
+        hipLaunchKernelGGL(commit_kernel, rgrid, rblock, 0, c->stream, a);
+        return;
     }
+#define PLIS_LAUNCH(V, MS, MR)                                                                        \
+    case V:                                                                                            \
+        if (which == 0) hipLaunchKernelGGL((phase_a_kernel<MS, MR>), grid, block, 0, c->stream, a);     \
+        else if (which == 1) hipLaunchKernelGGL((decide_kernel<MS, MR>), rgrid, rblock, 0, c->stream, a); \
+        else hipLaunchKernelGGL((phase_b_kernel<MS, MR>), grid, block, 0, c->stream, a);                \
+        break;
+    switch (c->variant) {
+        PLIS_LAUNCH(0, 1, 4)
+        PLIS_LAUNCH(1, 3, 4)
+        PLIS_LAUNCH(2, 5, 8)
+        PLIS_LAUNCH(3, 8, 16)
+        default:
+        PLIS_LAUNCH(4, 64, 128)
+    }
+#undef PLIS_LAUNCH
+}
+
+static void launch_step(plis_ctx* c, const KernelArgs& a) {
+    for (int k = 0; k < 4; ++k) launch_kernel(c, k, a);
 }
 
 static void set_shape(plis_ctx* c, uint32_t wpb, uint32_t bpc) {
@@ -292,10 +311,7 @@ static void set_shape(plis_ctx* c, uint32_t wpb, uint32_t bpc) {
 
 static int launch_iters(plis_ctx* c, uint32_t n) {
     const KernelArgs a = make_args(c);
-    for (uint32_t i = 0; i < n; ++i) {
-        launch_phase(c, 0, a);
-        launch_phase(c, 1, a);
-    }
+    for (uint32_t i = 0; i < n; ++i) launch_step(c, a);
     HIPCHK(c, hipGetLastError());
     return 0;
 }
@@ -310,10 +326,7 @@ static int graph_iters(plis_ctx* c, uint32_t n) {
         hipGraph_t g;
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         const KernelArgs a = make_args(c);
-        for (uint32_t i = 0; i < n; ++i) {
-            launch_phase(c, 0, a);
-            launch_phase(c, 1, a);
-        }
+        for (uint32_t i = 0; i < n; ++i) launch_step(c, a);
         HIPCHK(c, hipStreamEndCapture(c->stream, &g));
         HIPCHK(c, hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0));
         HIPCHK(c, hipGraphDestroy(g));
@@ -377,7 +390,9 @@ extern "C" int plis_create(const plis_network* net, int32_t n_real, const plis_t
     c->S = nh.S;
     c->R = nh.R;
     c->C = nh.C;
-    c->variant = (nh.S <= 1 && nh.R <= 4) ? 0 : (nh.S <= 8 && nh.R <= 16) ? 1 : 2;
+    c->variant = 4;
+    for (int v = 3; v >= 0; --v)
+        if (nh.S <= kVariantS[v] && nh.R <= kVariantR[v]) c->variant = v;
     c->eps = params->epsilon;
     c->nc = params->n_critical;
     c->seed = params->seed;
@@ -601,35 +616,31 @@ extern "C" int plis_iterate(plis_ctx* c, uint32_t n) {
 
 extern "C" int plis_profile(plis_ctx* c, uint32_t n, double* out) {
     if (!c || !out) return PLIS_ERR_ARG;
-    std::vector<hipEvent_t> ev(4 * (size_t)n + 2);
+    std::vector<hipEvent_t> ev(8 * (size_t)n + 2);
     for (auto& e : ev) HIPCHK(c, hipEventCreate(&e));
     const KernelArgs a = make_args(c);
     HIPCHK(c, hipEventRecord(ev[0], c->stream));
-    for (uint32_t i = 0; i < n; ++i) {
-        HIPCHK(c, hipEventRecord(ev[1 + 4 * i], c->stream));
-        launch_phase(c, 0, a);
-        HIPCHK(c, hipEventRecord(ev[2 + 4 * i], c->stream));
-        HIPCHK(c, hipEventRecord(ev[3 + 4 * i], c->stream));
-        launch_phase(c, 1, a);
-        HIPCHK(c, hipEventRecord(ev[4 + 4 * i], c->stream));
-    }
-    HIPCHK(c, hipEventRecord(ev[4 * n + 1], c->stream));
+    for (uint32_t i = 0; i < n; ++i)
+        for (int k = 0; k < 4; ++k) {
+            HIPCHK(c, hipEventRecord(ev[1 + 8 * i + 2 * k], c->stream));
+            launch_kernel(c, k, a);
+            HIPCHK(c, hipEventRecord(ev[2 + 8 * i + 2 * k], c->stream));
+        }
+    HIPCHK(c, hipEventRecord(ev[8 * n + 1], c->stream));
     HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipEventSynchronize(ev[4 * n + 1]));
+    HIPCHK(c, hipEventSynchronize(ev[8 * n + 1]));
     float tot = 0.f;
-    double ta = 0.0, tb = 0.0;
-    HIPCHK(c, hipEventElapsedTime(&tot, ev[0], ev[4 * n + 1]));
-    for (uint32_t i = 0; i < n; ++i) {
-        float x = 0.f, y = 0.f;
-        HIPCHK(c, hipEventElapsedTime(&x, ev[1 + 4 * i], ev[2 + 4 * i]));
-        HIPCHK(c, hipEventElapsedTime(&y, ev[3 + 4 * i], ev[4 + 4 * i]));
-        ta += x;
-        tb += y;
-    }
-    for (auto& e : ev) hipEventDestroy(e);
+    double t[4] = {0, 0, 0, 0};
+    HIPCHK(c, hipEventElapsedTime(&tot, ev[0], ev[8 * n + 1]));
+    for (uint32_t i = 0; i < n; ++i)
+        for (int k = 0; k < 4; ++k) {
+            float x = 0.f;
+            HIPCHK(c, hipEventElapsedTime(&x, ev[1 + 8 * i + 2 * k], ev[2 + 8 * i + 2 * k]));
+            t[k] += x;
+        }
+    for (auto& e : ev) (void)hipEventDestroy(e);
     out[0] = tot;
-    out[1] = ta;
-    out[2] = tb;
+    for (int k = 0; k < 4; ++k) out[1 + k] = t[k];
     return PLIS_OK;
 }
 
@@ -684,7 +695,7 @@ extern "C" int plis_debug_tau(plis_ctx* c, double* tp, double* a0c, double* t2) 
     a.dbg_tp = d_tp;
     a.dbg_a0c = d_a0c;
     a.dbg_t2 = d_t2;
-    launch_phase(c, 0, a);
+    launch_kernel(c, 0, a);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e == hipSuccess) e = hipMemcpy(tp, d_tp, n * sizeof(double), hipMemcpyDeviceToHost);
@@ -715,14 +726,14 @@ extern "C" int plis_autotune(plis_ctx* c, uint32_t iters) {
     const uint32_t bpcs[] = {1, 2, 4, 8, 16};
     double best = 1e300;
     uint32_t bw = c->wpb, bb = 4;
-    std::vector<double> prof(3);
+    std::vector<double> prof(5);
     for (uint32_t w : wpbs)
         for (uint32_t b : bpcs) {
             set_shape(c, w, b);
             if (int rc = plis_profile(c, 1, prof.data())) return rc;  // warm
             if (int rc = plis_profile(c, iters, prof.data())) return rc;
-            if (prof[1] + prof[2] < best) {
-                best = prof[1] + prof[2];
+            if (prof[1] + prof[3] < best) {
+                best = prof[1] + prof[3];
                 bw = w;
                 bb = b;
             }

@@ -4,17 +4,17 @@
 //   phase A (tau): per subvolume FP64 mass-action propensities, Cao tau' with
 //     hops treated as first-order channels, the critical set and an
 //     exponential tau'' clock; a wave-shuffle min/argmin per 64-subvolume
-//     tile, then a last-arriver reduction over groups of 64 tiles and over the
-//     realization's groups.  The last wave decides the step (tau, critical
-//     event) for its realization.  Replaces compute_rates
+//     tile written as a tile partial; a one-block-per-realization decide
+//     kernel reduces the partials and decides the step (tau, critical event).  Replaces compute_rates
 //     (PLISCuda/src/rates.cu:82-96), compute_taus (tau.cu:197-251) and the
 //     host thrust::min_element + tau[i] round trips (driver.cu:267-277).
 //   phase B (leap): per subvolume Poisson firings of non-critical reactions,
 //     gather-form diffusion (each subvolume regenerates its in-neighbours'
 //     hop counts from the same counter-based Philox draw the sender used, so
 //     there are no atomics on counts), the critical event of the argmin
-//     subvolume, and the negative-population check; the last wave commits the
-//     step or leaps back (tau' halved, new attempt index).  Replaces
+//     subvolume, and the negative-population check; a one-block-per-
+//     realization commit kernel accepts the step or leaps back (tau' halved,
+//     new attempt index).  Replaces
 //     leap_step (leap.cu:3-201), ssa_step (ssa.cu:56-110), check_state +
 //     none_of + revert (leap.cu:203-212, driver.cu:292-323).
 //
@@ -381,51 +381,48 @@ __global__ __launch_bounds__(256) void phase_a_kernel(KernelArgs a) {
         }
         if (dbg) continue;
         reduce_partA(tp, t2, v2);
-        const uint32_t g = ri.group_begin + (t - ri.tile_begin) / kGroup;
-        const uint32_t gfirst = ri.tile_begin + (g - ri.group_begin) * kGroup;
-        const uint32_t ng = min((uint32_t)kGroup, ri.tile_begin + ri.ntiles - gfirst);
-        int last = 0;
-        if (lane == 0) {
-            a.partA_tile[t] = PartA{tp, t2, v2, 0u};
-            __threadfence();
-            last = atomicAdd(&a.cntA_grp[g], 1u) == ng - 1u;
-        }
-        last = __shfl(last, 0);
-        if (!last) continue;
-        __threadfence();
-        {
-            PartA p = lane < ng ? a.partA_tile[gfirst + lane] : PartA{kInf, kInf, 0xFFFFFFFFu, 0u};
-            tp = p.tp;
+        if (lane == 0) a.partA_tile[t] = PartA{tp, t2, v2, 0u};
+    }
+}
+
+// One block per realization: min tau' / argmin tau'' over the realization's
+// tile partials (written by phase A; the kernel boundary makes them visible),
+// then the step decision.
+template <int MS, int MR>
+__global__ __launch_bounds__(256) void decide_kernel(KernelArgs a) {
+    const uint32_t r = blockIdx.x;
+    if (r >= a.nreal) return;
+    const RealInfo ri = a.info[r];
+    if (a.ctl[r].done || a.ctl[r].retry) return;
+    __shared__ double s_tp[4], s_t2[4];
+    __shared__ uint32_t s_v2[4];
+    double tp = kInf, t2 = kInf;
+    uint32_t v2 = 0xFFFFFFFFu;
+    for (uint32_t q = threadIdx.x; q < ri.ntiles; q += blockDim.x) {
+        const PartA p = a.partA_tile[ri.tile_begin + q];
+        tp = fmin(tp, p.tp);
+        if (p.t2 < t2 || (p.t2 == t2 && p.v2 < v2)) {
             t2 = p.t2;
             v2 = p.v2;
         }
-        reduce_partA(tp, t2, v2);
-        int lastr = 0;
-        if (lane == 0) {
-            a.partA_grp[g] = PartA{tp, t2, v2, 0u};
-            a.cntA_grp[g] = 0u;
-            __threadfence();
-            lastr = atomicAdd(&a.cntA_real[r], 1u) == ri.ngroups - 1u;
-        }
-        lastr = __shfl(lastr, 0);
-        if (!lastr) continue;
-        __threadfence();
-        tp = kInf;
-        t2 = kInf;
-        v2 = 0xFFFFFFFFu;
-        for (uint32_t q = lane; q < ri.ngroups; q += 64u) {
-            const PartA p = a.partA_grp[ri.group_begin + q];
-            tp = fmin(tp, p.tp);
-            if (p.t2 < t2 || (p.t2 == t2 && p.v2 < v2)) {
-                t2 = p.t2;
-                v2 = p.v2;
+    }
+    reduce_partA(tp, t2, v2);
+    const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    if (lane == 0) {
+        s_tp[wv] = tp;
+        s_t2[wv] = t2;
+        s_v2[wv] = v2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (uint32_t w = 1; w < (blockDim.x >> 6); ++w) {
+            tp = fmin(tp, s_tp[w]);
+            if (s_t2[w] < t2 || (s_t2[w] == t2 && s_v2[w] < v2)) {
+                t2 = s_t2[w];
+                v2 = s_v2[w];
             }
         }
-        reduce_partA(tp, t2, v2);
-        if (lane == 0) {
-            a.cntA_real[r] = 0u;
-            finalize_A<MS, MR>(a, n, ri, r, tp, t2, v2);
-        }
+        finalize_A<MS, MR>(a, net_view(a), ri, r, tp, t2, v2);
     }
 }
 
@@ -684,55 +681,52 @@ __global__ __launch_bounds__(256) void phase_b_kernel(KernelArgs a) {
         uint32_t bad = 0;
         if (v < ri.N) bad = leap_subvolume<MS, MR>(a, n, ri, X, Xn, v, sv, fire) ? 1u : 0u;
         reduce_partB(fire, bad);
-        const uint32_t g = ri.group_begin + (t - ri.tile_begin) / kGroup;
-        const uint32_t gfirst = ri.tile_begin + (g - ri.group_begin) * kGroup;
-        const uint32_t ng = min((uint32_t)kGroup, ri.tile_begin + ri.ntiles - gfirst);
-        int last = 0;
-        if (lane == 0) {
-            a.partB_tile[t] = PartB{fire, bad, 0u};
-            __threadfence();
-            last = atomicAdd(&a.cntB_grp[g], 1u) == ng - 1u;
-        }
-        last = __shfl(last, 0);
-        if (!last) continue;
-        __threadfence();
-        {
-            const PartB p = lane < ng ? a.partB_tile[gfirst + lane] : PartB{0ull, 0u, 0u};
-            fire = p.fire;
-            bad = p.bad;
-        }
-        reduce_partB(fire, bad);
-        int lastr = 0;
-        if (lane == 0) {
-            a.partB_grp[g] = PartB{fire, bad, 0u};
-            a.cntB_grp[g] = 0u;
-            __threadfence();
-            lastr = atomicAdd(&a.cntB_real[r], 1u) == ri.ngroups - 1u;
-        }
-        lastr = __shfl(lastr, 0);
-        if (!lastr) continue;
-        __threadfence();
-        fire = 0;
-        bad = 0;
-        for (uint32_t q = lane; q < ri.ngroups; q += 64u) {
-            const PartB p = a.partB_grp[ri.group_begin + q];
-            fire += p.fire;
-            bad |= p.bad;
-        }
-        reduce_partB(fire, bad);
-        if (lane == 0) {
-            a.cntB_real[r] = 0u;
-            finalize_B(a, ri, r, fire, bad);
-        }
+        if (lane == 0) a.partB_tile[t] = PartB{fire, bad, 0u};
     }
 }
 
-// Explicit instantiations: Schlogl-like (1 species, <= 4 reactions), small
-// networks (<= 8 species, <= 16 reactions), and the general case.
+// One block per realization: sum of firings / any negative over the tile
+// partials of phase B, then commit the step or leap back.
+__global__ __launch_bounds__(256) void commit_kernel(KernelArgs a) {
+    const uint32_t r = blockIdx.x;
+    if (r >= a.nreal) return;
+    const RealInfo ri = a.info[r];
+    if (a.ctl[r].done) return;
+    __shared__ unsigned long long s_f[4];
+    __shared__ uint32_t s_b[4];
+    unsigned long long fire = 0;
+    uint32_t bad = 0;
+    for (uint32_t q = threadIdx.x; q < ri.ntiles; q += blockDim.x) {
+        const PartB p = a.partB_tile[ri.tile_begin + q];
+        fire += p.fire;
+        bad |= p.bad;
+    }
+    reduce_partB(fire, bad);
+    const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    if (lane == 0) {
+        s_f[wv] = fire;
+        s_b[wv] = bad;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (uint32_t w = 1; w < (blockDim.x >> 6); ++w) {
+            fire += s_f[w];
+            bad |= s_b[w];
+        }
+        finalize_B(a, ri, r, fire, bad);
+    }
+}
+
+// Explicit instantiations, smallest fitting one is used: Schlogl-like
+// (1 species, <= 4 reactions), <= 3 x 4, <= 5 x 8 (Brusselator), <= 8 x 16
+// and the general case.
 #define PLIS_INST(MS, MR)                                                   \
     template __global__ void phase_a_kernel<MS, MR>(KernelArgs);            \
+    template __global__ void decide_kernel<MS, MR>(KernelArgs);             \
     template __global__ void phase_b_kernel<MS, MR>(KernelArgs);
 PLIS_INST(1, 4)
+PLIS_INST(3, 4)
+PLIS_INST(5, 8)
 PLIS_INST(8, 16)
 PLIS_INST(64, 128)
 

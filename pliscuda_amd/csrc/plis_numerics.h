@@ -44,12 +44,12 @@ __device__ __forceinline__ U4 philox10(uint32_t c0, uint32_t c1, uint32_t c2, ui
                                        uint32_t k0, uint32_t k1) {
 #pragma unroll
     for (int i = 0; i < 10; ++i) {
-        const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
-        const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
-        c0 = hi1 ^ c1 ^ k0;
-        c2 = hi0 ^ c3 ^ k1;
-        c1 = lo1;
-        c3 = lo0;
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0;   // one v_mad_u64_u32 each
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        c0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        c2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
         k0 += 0x9E3779B9u;
         k1 += 0xBB67AE85u;
     }
@@ -180,9 +180,10 @@ __device__ __attribute__((noinline)) long long poisson_ptrs(double lam, DrawId i
     return (long long)__builtin_floor(lam);
 }
 
-// Poisson(lam) from the packed word u0 (inversion) or the draw's PTRS stream.
-__device__ __forceinline__ long long poisson(double lam, uint32_t u0, const DrawId& id) {
-    if (!(lam > 0.0)) return 0;
+// Slow path of the sampler: full sequential-search inversion (mean < 10) or
+// PTRS (mean >= 10).  Kept out of line so the common no-event case below
+// stays a handful of instructions at every call site.
+__device__ __attribute__((noinline)) long long poisson_slow(double lam, uint32_t u0, DrawId id) {
     if (lam < kPoissonInvMax) {
         const double u = uniform01(u0);
         double p = dexp(-lam);
@@ -196,6 +197,16 @@ __device__ __forceinline__ long long poisson(double lam, uint32_t u0, const Draw
         return k;
     }
     return poisson_ptrs(lam, id);
+}
+
+// Poisson(lam) from the packed word u0 (inversion) or the draw's PTRS stream.
+// Fast exit: 1 - lam <= e^-lam, and the computed exp is within 2^-51 of
+// e^-lam, so u <= (1 - lam) - 2^-50 implies u <= F_0 and the inversion
+// returns 0 -- the same value the full search gives, without the exp.
+__device__ __forceinline__ long long poisson(double lam, uint32_t u0, const DrawId& id) {
+    if (!(lam > 0.0)) return 0;
+    if (lam < kPoissonInvMax && uniform01(u0) <= (1.0 - lam) - 0x1p-50) return 0;
+    return poisson_slow(lam, u0, id);
 }
 
 }  // namespace plis

@@ -1,0 +1,110 @@
+"""Pins the oracle's numerics: Philox against published / independent KAT
+vectors, exp/log/sqrt/log k! against the mathematics, the Poisson sampler
+against exact quantiles and goodness of fit."""
+import ctypes as C
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+from scipy import stats
+
+from oracle import ffi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+# Random123 kat_vectors (Salmon et al. 2011), philox4x32_10
+RANDOM123_KAT = [
+    ([0, 0, 0, 0], [0, 0], [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]),
+    ([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2, [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]),
+    ([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0],
+     [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]),
+]
+
+
+def test_philox_published_kat():
+    for ctr, key, out in RANDOM123_KAT:
+        assert ffi.philox(ctr, key) == out
+
+
+def test_philox_matches_rocrand_vectors():
+    with open(os.path.join(HERE, "golden", "philox_kat.json")) as f:
+        kat = json.load(f)
+    assert len(kat) >= 20
+    for k in kat:
+        assert ffi.philox(k["ctr"], k["key"]) == k["out"]
+
+
+def test_exp_log_sqrt_accuracy():
+    L = ffi.lib()
+    xs = np.concatenate([-np.linspace(0, 12, 4001), -np.geomspace(1e-15, 700, 800), np.linspace(0, 5, 101)])
+    for x in xs:
+        e = L.orc_exp(float(x))
+        assert abs(e - math.exp(x)) <= 4.5e-16 * math.exp(x), x
+    ys = np.concatenate([np.geomspace(1e-300, 1e300, 4000), np.linspace(0.25, 4, 2001)])
+    for y in ys:
+        l = L.orc_log(float(y))
+        assert abs(l - math.log(y)) <= 4.5e-16 * max(abs(math.log(y)), 1e-2), y
+    for y in np.geomspace(1.0, 1e15, 1000):
+        assert abs(L.orc_sqrt(float(y)) - math.sqrt(y)) <= 4.5e-16 * math.sqrt(y)
+    assert L.orc_log(1.0) == 0.0 and L.orc_exp(0.0) == 1.0
+
+
+def test_logfact_matches_lgamma():
+    L = ffi.lib()
+    for k in list(range(0, 40)) + [100, 1000, 10**5, 10**8]:
+        assert abs(L.orc_logfact(k) - math.lgamma(k + 1)) <= 1e-10 * max(1.0, math.lgamma(k + 1))
+
+
+def test_zero_event_shortcut_bound():
+    """The kernels skip the exp when u <= (1 - lam) - 2^-50; that is exact only
+    if this bound never exceeds the oracle's computed e^-lam."""
+    L = ffi.lib()
+    lams = np.concatenate([np.geomspace(1e-300, 1.0, 20000), np.linspace(0, 1, 20001)[1:]])
+    for lam in lams:
+        assert (1.0 - lam) - 2.0 ** -50 <= L.orc_exp(-float(lam)), lam
+
+
+def _poisson_from_word(lam, u0):
+    L = ffi.lib()
+    ctr = (C.c_uint32 * 4)(0, 0, 0, 0)
+    key = (C.c_uint32 * 2)(0, 0)
+    L.orc_poisson.argtypes = [C.c_double, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                              C.c_uint32]
+    L.orc_poisson.restype = C.c_int64
+    return L.orc_poisson(lam, u0, ctr, key, 0)
+
+
+@pytest.mark.parametrize("lam", [0.01, 0.37, 1.0, 2.5, 6.0, 9.99])
+def test_inversion_equals_exact_quantile(lam):
+    """Small means: the draw is the Poisson quantile of u = (w + 0.5) 2^-32."""
+    rng = np.random.default_rng(int(lam * 1000))
+    words = rng.integers(0, 2**32, size=3000, dtype=np.uint64)
+    cdf = stats.poisson.cdf(np.arange(0, 200), lam)
+    for w in words:
+        u = (float(w) + 0.5) * 2.0 ** -32
+        k = _poisson_from_word(lam, int(w))
+        # skip words within 1e-12 of a CDF step (rounding of the running sum)
+        if np.min(np.abs(cdf - u)) < 1e-12:
+            continue
+        assert k == int(np.searchsorted(cdf, u)), (lam, u)
+
+
+@pytest.mark.parametrize("lam", [0.05, 0.8, 4.0, 9.9, 10.0, 12.5, 40.0, 300.0, 20000.0])
+def test_poisson_goodness_of_fit(lam):
+    n = 200000
+    s = ffi.poisson_many(lam, 12345 + int(lam * 7), n)
+    se = math.sqrt(lam / n)
+    assert abs(s.mean() - lam) < 4.5 * se
+    var_se = lam * math.sqrt(2.0 / n) * math.sqrt(1 + 1 / (2 * lam))
+    assert abs(s.var() - lam) < 4.5 * var_se
+    lo, hi = int(max(0, lam - 6 * math.sqrt(lam))), int(lam + 6 * math.sqrt(lam) + 6)
+    ks = np.arange(lo, hi + 1)
+    obs = np.array([(s == k).sum() for k in ks], dtype=float)
+    exp = stats.poisson.pmf(ks, lam) * n
+    keep = exp > 20
+    obs, exp = obs[keep], exp[keep]
+    chi2 = ((obs - exp) ** 2 / exp).sum()
+    p = stats.chi2.sf(chi2, keep.sum() - 1)
+    assert p > 1e-4, (lam, chi2, p)
