@@ -53,6 +53,7 @@ struct plis_ctx {
     uint64_t seed = 0;
     uint32_t wpb = 4, blocks = 8;
     int use_graph = 1;
+    bool edge_parallel = false;  // every realization symmetric: edge-parallel leap tiles
     hipGraphExec_t graph = nullptr;
     uint32_t graph_iters = 0;
     std::vector<void*> allocs;
@@ -279,7 +280,9 @@ static void launch_kernel(plis_ctx* c, int which, const KernelArgs& a) {
     case V:                                                                                            \
         if (which == 0) hipLaunchKernelGGL((phase_a_kernel<MS, MR>), grid, block, 0, c->stream, a);     \
         else if (which == 1) hipLaunchKernelGGL((decide_kernel<MS, MR>), rgrid, rblock, 0, c->stream, a); \
-        else hipLaunchKernelGGL((phase_b_kernel<MS, MR>), grid, block, 0, c->stream, a);                \
+        else if (c->edge_parallel && MS <= 8)                                                          \
+            hipLaunchKernelGGL((phase_b_kernel<MS, MR, (MS <= 8)>), grid, block, 0, c->stream, a);      \
+        else hipLaunchKernelGGL((phase_b_kernel<MS, MR, false>), grid, block, 0, c->stream, a);         \
         break;
     switch (c->variant) {
         PLIS_LAUNCH(0, 1, 4)
@@ -509,7 +512,7 @@ extern "C" int plis_create(const plis_network* net, int32_t n_real, const plis_t
         plis_destroy(c);
         return rc;
     }
-    (void)any_asym;
+    c->edge_parallel = !any_asym;
     if (hipHostMalloc((void**)&c->h_ndone, sizeof(uint32_t)) != hipSuccess) {
         g_create_err = "hipHostMalloc failed";
         plis_destroy(c);

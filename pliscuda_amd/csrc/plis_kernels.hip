@@ -398,12 +398,21 @@ __global__ __launch_bounds__(256) void decide_kernel(KernelArgs a) {
     __shared__ uint32_t s_v2[4];
     double tp = kInf, t2 = kInf;
     uint32_t v2 = 0xFFFFFFFFu;
-    for (uint32_t q = threadIdx.x; q < ri.ntiles; q += blockDim.x) {
-        const PartA p = a.partA_tile[ri.tile_begin + q];
-        tp = fmin(tp, p.tp);
-        if (p.t2 < t2 || (p.t2 == t2 && p.v2 < v2)) {
-            t2 = p.t2;
-            v2 = p.v2;
+    // 8 independent loads in flight per thread
+    for (uint32_t q0 = 0; q0 < ri.ntiles; q0 += 8u * blockDim.x) {
+        PartA p[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const uint32_t q = q0 + (uint32_t)i * blockDim.x + threadIdx.x;
+            p[i] = q < ri.ntiles ? a.partA_tile[ri.tile_begin + q] : PartA{kInf, kInf, 0xFFFFFFFFu, 0u};
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            tp = fmin(tp, p[i].tp);
+            if (p[i].t2 < t2 || (p[i].t2 == t2 && p[i].v2 < v2)) {
+                t2 = p[i].t2;
+                v2 = p[i].v2;
+            }
         }
     }
     reduce_partA(tp, t2, v2);
@@ -491,49 +500,54 @@ __device__ __forceinline__ bool leap_subvolume(const KernelArgs& a, const NetVie
     }
     const uint32_t c3hop = ctr_word3(sv.real, sv.attempt, kHop, 0u);
     if (sym) {
-        // one Philox block per (edge, species pair) serves both directions
-        for (uint32_t q = rb; q < re; ++q) {
-            const uint32_t u = a.col[ri.e_off + q];
-            const double wq = weighted ? (double)a.w[ri.e_off + q] : 1.0;
-            const int cu = n.C > 1 ? a.comp[ri.comp_off + u] : 0;
-            const uint32_t lo = v < u ? v : u, hi = v < u ? u : v;
-            const uint32_t dout = v > u ? 1u : 0u, din = 1u - dout;
+        // One Philox block per (edge, species pair) serves both directions of
+        // both species.  Pairs outer (unrolled: static indices keep the count
+        // accumulators in registers), edges inner (rolled: few live values).
+        const uint32_t* colp = a.col + ri.e_off;
+        const float* wp = a.w + ri.e_off;
 #pragma unroll
-            for (int s0 = 0; s0 < MS; s0 += 2) {
-                if (s0 >= n.S) break;
-                double lout[2], lin[2];
-                bool any = false;
-#pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const int s = s0 + h;
-                    lout[h] = 0.0;
-                    lin[h] = 0.0;
-                    if (s >= MS || s >= n.S) continue;
-                    if (act[s]) lout[h] = tau * (dx[s] * wq);
-                    const uint32_t xu = X[ri.x_off + (uint64_t)s * N + u];
-                    const double du = n.d[cu * n.S + s];
-                    if (hop_active(du, xu, nc)) lin[h] = tau * ((du * (double)xu) * wq);
-                    any = any || lout[h] > 0.0 || lin[h] > 0.0;
+        for (int s0 = 0; s0 < MS; s0 += 2) {
+            if (s0 >= n.S) break;
+            const bool two = s0 + 1 < MS && s0 + 1 < n.S;
+            const bool act0 = act[s0], act1 = two && act[s0 + 1 < MS ? s0 + 1 : s0];
+            const double dx0 = dx[s0], dx1 = two ? dx[s0 + 1 < MS ? s0 + 1 : s0] : 0.0;
+            const uint32_t* X0 = X + ri.x_off + (uint64_t)s0 * N;
+            const uint32_t* X1 = two ? X0 + N : X0;
+            const uint32_t cb = c3hop | ((uint32_t)s0 >> 1);
+            long long d0 = 0, d1 = 0;
+            for (uint32_t q = rb; q < re; ++q) {
+                const uint32_t u = colp[q];
+                const double wq = weighted ? (double)wp[q] : 1.0;
+                const int cu = n.C > 1 ? a.comp[ri.comp_off + u] : 0;
+                const uint32_t xu0 = X0[u];
+                const uint32_t xu1 = two ? X1[u] : 0u;
+                const double du0 = n.d[cu * n.S + s0];
+                const double du1 = two ? n.d[cu * n.S + s0 + 1] : 0.0;
+                const double lo0 = act0 ? tau * (dx0 * wq) : 0.0;
+                const double lo1 = act1 ? tau * (dx1 * wq) : 0.0;
+                const double li0 = hop_active(du0, xu0, nc) ? tau * ((du0 * (double)xu0) * wq) : 0.0;
+                const double li1 = (two && hop_active(du1, xu1, nc)) ? tau * ((du1 * (double)xu1) * wq) : 0.0;
+                if (!(lo0 > 0.0 || lo1 > 0.0 || li0 > 0.0 || li1 > 0.0)) continue;
+                const uint32_t lo = v < u ? v : u, hi = v < u ? u : v;
+                const uint32_t dout = v > u ? 1u : 0u, din = 1u - dout;
+                const U4 o = philox10(lo, hi, sv.step, cb, k0, k1);
+                if (lo0 > 0.0) {
+                    const long long k = poisson(lo0, pick(o, dout), DrawId{lo, hi, sv.step, cb, k0, k1, dout});
+                    d0 -= k;
+                    fire += (unsigned long long)k;
                 }
-                if (!any) continue;
-                const uint32_t b = (uint32_t)s0 >> 1;
-                const U4 o = philox10(lo, hi, sv.step, c3hop | b, k0, k1);
-#pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const int s = s0 + h;
-                    if (s >= MS || s >= n.S) continue;
-                    const uint32_t wo = 2u * (uint32_t)h + dout, wi = 2u * (uint32_t)h + din;
-                    if (lout[h] > 0.0) {
-                        const DrawId id{lo, hi, sv.step, c3hop | b, k0, k1, wo};
-                        const long long k = poisson(lout[h], pick(o, wo), id);
-                        nx[s] -= k;
-                        fire += (unsigned long long)k;
-                    }
-                    if (lin[h] > 0.0) {
-                        const DrawId id{lo, hi, sv.step, c3hop | b, k0, k1, wi};
-                        nx[s] += poisson(lin[h], pick(o, wi), id);
-                    }
+                if (li0 > 0.0) d0 += poisson(li0, pick(o, din), DrawId{lo, hi, sv.step, cb, k0, k1, din});
+                if (lo1 > 0.0) {
+                    const long long k = poisson(lo1, pick(o, 2u + dout), DrawId{lo, hi, sv.step, cb, k0, k1, 2u + dout});
+                    d1 -= k;
+                    fire += (unsigned long long)k;
                 }
+                if (li1 > 0.0) d1 += poisson(li1, pick(o, 2u + din), DrawId{lo, hi, sv.step, cb, k0, k1, 2u + din});
+            }
+#pragma unroll
+            for (int q = 0; q < MS; ++q) {
+                if (q == s0) nx[q] += d0;
+                if (q == s0 + 1) nx[q] += d1;
             }
         }
     } else {
@@ -607,6 +621,171 @@ __device__ __forceinline__ bool leap_subvolume(const KernelArgs& a, const NetVie
     return bad;
 }
 
+// Reactions, the critical event and the negativity check of one subvolume
+// whose hop balance (in-hops minus out-hops) is already known.
+template <int MS, int MR>
+__device__ __forceinline__ bool finish_subvolume(const KernelArgs& a, const NetView& n, const RealInfo& ri,
+                                                 uint32_t* Xn, uint32_t v, const uint32_t (&x)[MS], int c,
+                                                 const long long (&hop)[MS], const StepView& sv,
+                                                 unsigned long long& fire) {
+    const uint32_t N = ri.N;
+    double av[MR];
+    uint32_t crit[(MR + 31) / 32];
+    react_props<MR, MS>(n, x, c, a.n_critical, av, crit);
+    long long nx[MS];
+#pragma unroll
+    for (int s = 0; s < MS; ++s) nx[s] = (long long)x[s] + hop[s];
+    const double tau = sv.tau;
+    const uint32_t k0 = a.seed_lo, k1 = a.seed_hi;
+    uint32_t blk = 0xFFFFFFFFu;
+    U4 o{0, 0, 0, 0};
+    const uint32_t c3base = ctr_word3(sv.real, sv.attempt, kReact, 0u);
+    for (int j = 0; j < n.R; ++j) {
+        if (is_crit<MR>(crit, j) || !(av[j] > 0.0)) continue;
+        const uint32_t b = (uint32_t)j >> 2;
+        if (b != blk) {
+            o = philox10(v, kSelfId, sv.step, c3base | b, k0, k1);
+            blk = b;
+        }
+        const uint32_t wd = (uint32_t)j & 3u;
+        const long long k = poisson(tau * av[j], pick(o, wd), DrawId{v, kSelfId, sv.step, c3base | b, k0, k1, wd});
+        fire += (unsigned long long)k;
+#pragma unroll
+        for (int s = 0; s < MS; ++s)
+            if (s < n.S) nx[s] += k * (long long)n.nu[s * n.R + j];
+    }
+    if (sv.crit) {
+        if (v == sv.vstar) {
+            if (sv.ev_kind == 1) {
+#pragma unroll
+                for (int s = 0; s < MS; ++s)
+                    if (s < n.S) nx[s] += n.nu[s * n.R + sv.ev_idx];
+                fire += 1ull;
+            } else if (sv.ev_kind == 2) {
+#pragma unroll
+                for (int s = 0; s < MS; ++s)
+                    if (s == sv.ev_idx) nx[s] -= 1;
+                fire += 1ull;
+            }
+        }
+        if (sv.ev_kind == 2 && v == sv.ev_dst && sv.ev_dst != sv.vstar) {
+#pragma unroll
+            for (int s = 0; s < MS; ++s)
+                if (s == sv.ev_idx) nx[s] += 1;
+        }
+    }
+    bool bad = false;
+#pragma unroll
+    for (int s = 0; s < MS; ++s) {
+        if (s >= n.S) break;
+        bad = bad || nx[s] < 0 || nx[s] > 0xFFFFFFFFll;
+        Xn[ri.x_off + (uint64_t)s * N + v] = (uint32_t)nx[s];
+    }
+    return bad;
+}
+
+// Edge-parallel leap of one tile of a symmetric realization.  The tile's
+// CSR edges are contiguous; lane l takes edges l, l+64, ... (coalesced
+// neighbour-id and weight loads, no divergence from uneven degrees), finds
+// the owning subvolume by binary search over the tile's row pointers staged
+// in LDS, draws both hop directions of every species from the edge's Philox
+// blocks and adds the owner's integer balance into LDS (integer sums are
+// order-free, so this is exact).  Then lane i finishes subvolume v0+i.
+template <int MS, int MR>
+__device__ __forceinline__ void leap_tile_edges(const KernelArgs& a, const NetView& n, const RealInfo& ri,
+                                                const uint32_t* X, uint32_t* Xn, uint32_t v0,
+                                                const StepView& sv, uint32_t (*s_x)[kTile],
+                                                unsigned long long (*s_del)[kTile], uint32_t* s_rp,
+                                                int32_t* s_c, unsigned long long& fire, uint32_t& bad) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t N = ri.N;
+    const int nc = a.n_critical;
+    const uint32_t v = v0 + lane;
+    const bool active = v < N;
+    uint32_t x[MS];
+#pragma unroll
+    for (int s = 0; s < MS; ++s) {
+        x[s] = (active && s < n.S) ? X[ri.x_off + (uint64_t)s * N + v] : 0u;
+        s_x[s][lane] = x[s];
+        s_del[s][lane] = 0ull;
+    }
+    const int c = (n.C > 1 && active) ? a.comp[ri.comp_off + v] : 0;
+    s_c[lane] = c;
+    s_rp[lane] = a.rp[ri.rp_off + min(v, N)];
+    if (lane == 0) s_rp[kTile] = a.rp[ri.rp_off + min(v0 + (uint32_t)kTile, N)];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+    const bool weighted = (ri.flags & 2u) != 0;
+    const uint32_t* colp = a.col + ri.e_off;
+    const float* wp = a.w + ri.e_off;
+    const double tau = sv.tau;
+    const uint32_t k0 = a.seed_lo, k1 = a.seed_hi;
+    const uint32_t c3hop = ctr_word3(sv.real, sv.attempt, kHop, 0u);
+    const uint32_t eb = s_rp[0], ee = s_rp[kTile];
+    for (uint32_t e = eb + lane; e < ee; e += (uint32_t)kTile) {
+        // owner: last i with s_rp[i] <= e
+        uint32_t lo_i = 0, hi_i = kTile - 1;
+#pragma unroll
+        for (int it = 0; it < 6; ++it) {
+            const uint32_t mid = (lo_i + hi_i + 1u) >> 1;
+            if (s_rp[mid] <= e) lo_i = mid;
+            else hi_i = mid - 1u;
+        }
+        const uint32_t i = lo_i;
+        const uint32_t vo = v0 + i;
+        const uint32_t u = colp[e];
+        const double wq = weighted ? (double)wp[e] : 1.0;
+        const int co = s_c[i];
+        const int cu = n.C > 1 ? a.comp[ri.comp_off + u] : 0;
+        const uint32_t lo = vo < u ? vo : u, hi = vo < u ? u : vo;
+        const uint32_t dout = vo > u ? 1u : 0u, din = 1u - dout;
+#pragma unroll
+        for (int s0 = 0; s0 < MS; s0 += 2) {
+            if (s0 >= n.S) break;
+            const bool two = s0 + 1 < MS && s0 + 1 < n.S;
+            const int s1 = s0 + 1 < MS ? s0 + 1 : s0;
+            const uint32_t xo0 = s_x[s0][i], xo1 = two ? s_x[s1][i] : 0u;
+            const double do0 = n.d[co * n.S + s0], do1 = two ? n.d[co * n.S + s1] : 0.0;
+            const uint32_t xu0 = X[ri.x_off + (uint64_t)s0 * N + u];
+            const uint32_t xu1 = two ? X[ri.x_off + (uint64_t)s1 * N + u] : 0u;
+            const double du0 = n.d[cu * n.S + s0], du1 = two ? n.d[cu * n.S + s1] : 0.0;
+            const double lo0 = hop_active(do0, xo0, nc) ? tau * ((do0 * (double)xo0) * wq) : 0.0;
+            const double lo1 = (two && hop_active(do1, xo1, nc)) ? tau * ((do1 * (double)xo1) * wq) : 0.0;
+            const double li0 = hop_active(du0, xu0, nc) ? tau * ((du0 * (double)xu0) * wq) : 0.0;
+            const double li1 = (two && hop_active(du1, xu1, nc)) ? tau * ((du1 * (double)xu1) * wq) : 0.0;
+            if (!(lo0 > 0.0 || lo1 > 0.0 || li0 > 0.0 || li1 > 0.0)) continue;
+            const uint32_t cb = c3hop | ((uint32_t)s0 >> 1);
+            const U4 o = philox10(lo, hi, sv.step, cb, k0, k1);
+            long long b0 = 0, b1 = 0;
+            if (lo0 > 0.0) {
+                const long long k = poisson(lo0, pick(o, dout), DrawId{lo, hi, sv.step, cb, k0, k1, dout});
+                b0 -= k;
+                fire += (unsigned long long)k;
+            }
+            if (li0 > 0.0) b0 += poisson(li0, pick(o, din), DrawId{lo, hi, sv.step, cb, k0, k1, din});
+            if (lo1 > 0.0) {
+                const long long k = poisson(lo1, pick(o, 2u + dout), DrawId{lo, hi, sv.step, cb, k0, k1, 2u + dout});
+                b1 -= k;
+                fire += (unsigned long long)k;
+            }
+            if (li1 > 0.0) b1 += poisson(li1, pick(o, 2u + din), DrawId{lo, hi, sv.step, cb, k0, k1, 2u + din});
+            if (b0 != 0) atomicAdd(&s_del[s0][i], (unsigned long long)b0);
+            if (b1 != 0) atomicAdd(&s_del[s1][i], (unsigned long long)b1);
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (active) {
+        long long hop[MS];
+#pragma unroll
+        for (int s = 0; s < MS; ++s) hop[s] = (long long)s_del[s][lane];
+        bad = finish_subvolume<MS, MR>(a, n, ri, Xn, v, x, c, hop, sv, fire) ? 1u : 0u;
+    }
+}
+
 __device__ __forceinline__ void reduce_partB(unsigned long long& fire, uint32_t& bad) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -651,10 +830,20 @@ __device__ void finalize_B(const KernelArgs& a, const RealInfo& ri, uint32_t r, 
     a.ctl[r] = c;
 }
 
-template <int MS, int MR>
+// EP: edge-parallel tiles (every realization of the context is symmetric);
+// otherwise one lane walks one subvolume's out- and in-edges.
+template <int MS, int MR, bool EP>
 __global__ __launch_bounds__(256) void phase_b_kernel(KernelArgs a) {
     const NetView n = net_view(a);
     const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wv = wave_uniform(threadIdx.x >> 6);
+    // LDS staging for the edge-parallel path (up to 4 waves per block)
+    constexpr bool kEdgePar = EP;
+    constexpr int LS = kEdgePar ? MS : 1;
+    __shared__ uint32_t s_x[4][LS][kTile];
+    __shared__ unsigned long long s_del[4][LS][kTile];
+    __shared__ uint32_t s_rp[4][kTile + 1];
+    __shared__ int32_t s_c[4][kTile];
     for (uint32_t it = 0;; ++it) {
         const uint32_t t = tile_at(it, a.ntiles, a.waves_per_block);
         if (t == 0xFFFFFFFFu) break;
@@ -679,7 +868,11 @@ __global__ __launch_bounds__(256) void phase_b_kernel(KernelArgs a) {
         const uint32_t v = te.v0 + lane;
         unsigned long long fire = 0;
         uint32_t bad = 0;
-        if (v < ri.N) bad = leap_subvolume<MS, MR>(a, n, ri, X, Xn, v, sv, fire) ? 1u : 0u;
+        if constexpr (kEdgePar) {
+            leap_tile_edges<MS, MR>(a, n, ri, X, Xn, te.v0, sv, s_x[wv], s_del[wv], s_rp[wv], s_c[wv], fire, bad);
+        } else {
+            if (v < ri.N) bad = leap_subvolume<MS, MR>(a, n, ri, X, Xn, v, sv, fire) ? 1u : 0u;
+        }
         reduce_partB(fire, bad);
         if (lane == 0) a.partB_tile[t] = PartB{fire, bad, 0u};
     }
@@ -696,10 +889,18 @@ __global__ __launch_bounds__(256) void commit_kernel(KernelArgs a) {
     __shared__ uint32_t s_b[4];
     unsigned long long fire = 0;
     uint32_t bad = 0;
-    for (uint32_t q = threadIdx.x; q < ri.ntiles; q += blockDim.x) {
-        const PartB p = a.partB_tile[ri.tile_begin + q];
-        fire += p.fire;
-        bad |= p.bad;
+    for (uint32_t q0 = 0; q0 < ri.ntiles; q0 += 8u * blockDim.x) {
+        PartB p[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const uint32_t q = q0 + (uint32_t)i * blockDim.x + threadIdx.x;
+            p[i] = q < ri.ntiles ? a.partB_tile[ri.tile_begin + q] : PartB{0ull, 0u, 0u};
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            fire += p[i].fire;
+            bad |= p[i].bad;
+        }
     }
     reduce_partB(fire, bad);
     const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63u;
@@ -723,11 +924,16 @@ __global__ __launch_bounds__(256) void commit_kernel(KernelArgs a) {
 #define PLIS_INST(MS, MR)                                                   \
     template __global__ void phase_a_kernel<MS, MR>(KernelArgs);            \
     template __global__ void decide_kernel<MS, MR>(KernelArgs);             \
-    template __global__ void phase_b_kernel<MS, MR>(KernelArgs);
+    template __global__ void phase_b_kernel<MS, MR, false>(KernelArgs);
+#define PLIS_INST_EP(MS, MR) template __global__ void phase_b_kernel<MS, MR, true>(KernelArgs);
 PLIS_INST(1, 4)
 PLIS_INST(3, 4)
 PLIS_INST(5, 8)
 PLIS_INST(8, 16)
 PLIS_INST(64, 128)
+PLIS_INST_EP(1, 4)
+PLIS_INST_EP(3, 4)
+PLIS_INST_EP(5, 8)
+PLIS_INST_EP(8, 16)
 
 }  // namespace plis

@@ -90,7 +90,7 @@ __device__ __forceinline__ double dexp(double x) {
 
 // log(x) for normal x > 0: x = m 2^e with m in (sqrt2/2, sqrt2],
 // log m = 2 atanh(s), s = (m-1)/(m+1), odd series through s^21.
-__device__ __forceinline__ double dlog(double x) {
+__device__ __attribute__((noinline)) double dlog(double x) {
     const unsigned long long b = (unsigned long long)__double_as_longlong(x);
     int e = (int)((b >> 52) & 0x7FF) - 1023;
     double m = __longlong_as_double((long long)((b & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull));
@@ -165,17 +165,26 @@ __device__ __attribute__((noinline)) long long poisson_ptrs(double lam, DrawId i
     const double invalpha = 1.1239 + 1.1328 / (b - 3.4);
     const double vr = 0.9277 - 3.6224 / (b - 2.0);
     const double loginvalpha = dlog(invalpha);
-    for (uint32_t it = 0; it < 100000u; ++it) {
-        const double U = extra_uniform(id, 2u * it) - 0.5;
-        const double V = extra_uniform(id, 2u * it + 1u);
-        const double us = 0.5 - __builtin_fabs(U);
-        const double kd = __builtin_floor((2.0 * a / us + b) * U + lam + 0.43);
-        if (us >= 0.07 && V <= vr) return (long long)kd;
-        if (kd < 0.0 || (us < 0.013 && V > us)) continue;
-        const long long k = (long long)kd;
-        const double lhs = (dlog(V) + loginvalpha) - dlog(a / (us * us) + b);
-        const double rhs = (-lam + kd * loglam) - dlogfact(k);
-        if (lhs <= rhs) return k;
+    const uint32_t c3 = (id.c3 & ~(7u << 7)) | (kExtra << 7);
+    // iteration it uses uniforms 2it, 2it+1 of the draw's stream; one Philox
+    // call (4 words) covers two iterations
+#pragma unroll 1
+    for (uint32_t call = 0; call < 50000u; ++call) {
+        const uint32_t k0 = id.k0 ^ (0x85EBCA6Bu * (4u * call + id.word + 1u));
+        const U4 o = philox10(id.c0, id.c1, id.c2, c3, k0, id.k1);
+#pragma unroll 1
+        for (uint32_t h = 0; h < 2u; ++h) {
+            const double U = uniform01(h ? o.z : o.x) - 0.5;
+            const double V = uniform01(h ? o.w : o.y);
+            const double us = 0.5 - __builtin_fabs(U);
+            const double kd = __builtin_floor((2.0 * a / us + b) * U + lam + 0.43);
+            if (us >= 0.07 && V <= vr) return (long long)kd;
+            if (kd < 0.0 || (us < 0.013 && V > us)) continue;
+            const long long k = (long long)kd;
+            const double lhs = (dlog(V) + loginvalpha) - dlog(a / (us * us) + b);
+            const double rhs = (-lam + kd * loglam) - dlogfact(k);
+            if (lhs <= rhs) return k;
+        }
     }
     return (long long)__builtin_floor(lam);
 }
@@ -189,6 +198,7 @@ __device__ __attribute__((noinline)) long long poisson_slow(double lam, uint32_t
         double p = dexp(-lam);
         double F = p;
         long long k = 0;
+#pragma unroll 1
         while (u > F && k < 1000) {
             k += 1;
             p = (p * lam) / (double)k;
