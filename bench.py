@@ -124,6 +124,18 @@ def cpu_baseline(wl, seed, sample_steps, gpu_check=True):
             "seconds": dt, "gpu_state_bit_identical": parity}
 
 
+def copy_probe(nbytes=1 << 28, reps=5):
+    """A plain device-to-device copy of known size (torch elementwise copy),
+    used to calibrate the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    a = torch.ones(nbytes // 4, dtype=torch.int32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": nbytes, "reps": reps}), flush=True)
+
+
 def load_traffic(kernel):
     try:
         with open(PMC_TRAFFIC) as f:
@@ -143,7 +155,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--extra", action="store_true", help="also measure the Schlogl grid and batch")
+    ap.add_argument("--copy-probe", action="store_true", help="only run the counter-calibration copy")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(realizations shard across GPUs by construction, see the design notes)",

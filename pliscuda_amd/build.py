@@ -35,7 +35,7 @@ def build_lib():
 def build_driver():
     os.makedirs(BIN_DIR, exist_ok=True)
     host = os.path.join(CSRC, "host")
-    srcs = [os.path.join(host, f) for f in ("driver.cpp", "main.cpp")]
+    srcs = [os.path.join(host, f) for f in ("plis_inputs.cpp", "driver.cpp", "main.cpp")]
     _run(["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), "-I", host,
           "-o", DRIVER + ".tmp", *srcs, "-L", HERE, "-lplis_hip", f"-Wl,-rpath,$ORIGIN/.."])
     os.replace(DRIVER + ".tmp", DRIVER)

@@ -145,6 +145,8 @@ def read_reactions(path):
 
 
 def read_constants(path, R, S):
+    """Constants are single precision in the reference (std::stof into float
+    arrays, cpp_utils.hpp:37,43); they are rounded to float32 here too."""
     with open(path) as f:
         l1 = f.readline()
         l2 = f.readline()
@@ -152,7 +154,8 @@ def read_constants(path, R, S):
     d = [float(t) for t in _split(l2)]
     if len(k) < R or len(d) < S:
         raise ValueError("constants file has too few values")
-    return np.array(k[:R]), np.array(d[:S])
+    return (np.array(k[:R], dtype=np.float32).astype(np.float64),
+            np.array(d[:S], dtype=np.float32).astype(np.float64))
 
 
 def read_compartments(path, N):

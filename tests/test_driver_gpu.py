@@ -1,0 +1,101 @@
+"""End-to-end: the `pliscuda conf.txt` driver on the reference's six test cases
+(PLISCuda/test/*), checked the way the reference's test/run.sh checks them
+(population totals from the final sim*.dat against golden/pops.txt) where
+the golden is an invariant, against every stoichiometric conservation law
+otherwise, and bit-for-bit against the oracle from the same seed."""
+import glob
+import os
+import re
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+from scipy.linalg import null_space
+
+from oracle import ffi
+from pliscuda_amd import inputs
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CASES = os.path.join(HERE, "golden", "ref_cases")
+DRIVER = os.path.join(os.path.dirname(HERE), "pliscuda_amd", "bin", "pliscuda")
+SEED = 4321
+
+
+def run_case(name, tmp_path, extra=""):
+    src = os.path.join(CASES, name)
+    for f in os.listdir(src):
+        if f.endswith(".txt"):
+            shutil.copy(os.path.join(src, f), tmp_path / f)
+    conf = (tmp_path / "conf.txt").read_text().rstrip("\n") + f"\nepsilon=0.05\nnc=10\nseed={SEED}\n{extra}"
+    (tmp_path / "conf.txt").write_text(conf)
+    print(f"driver case {name}", flush=True)
+    r = subprocess.run([DRIVER, "conf.txt"], cwd=tmp_path, capture_output=True, text=True, timeout=110)
+    assert r.returncode == 0, r.stdout + r.stderr
+    for key in ("final simulation time:", "elapsed time:", "simulation steps:", "steps/second:"):
+        assert key in r.stdout
+    snaps = sorted(glob.glob(str(tmp_path / "sim*.dat")))
+    assert snaps
+    final = max(snaps, key=lambda p: float(re.search(r"_([0-9.]+)\.dat$", p).group(1)))
+    lines = open(final).read().split("\n")
+    N, S = int(lines[0].split("=")[1]), int(lines[1].split("=")[1])
+    t = float(lines[2].split("=")[1])
+    x = np.array([[int(v) for v in lines[4 + i].split()] for i in range(N)], dtype=np.int64)
+    return r.stdout, t, x.T.copy(), final, snaps
+
+
+def conservation_ok(net, x0, x1):
+    S, R = net["S"], net["R"]
+    nu = (np.asarray(net["prod"]) - np.asarray(net["reac"])).reshape(S, R).astype(float)
+    L = null_space(nu.T)  # laws L^T nu = 0
+    if L.size == 0:
+        return True
+    a = L.T @ x0.reshape(S, -1).sum(1).astype(float)
+    b = L.T @ x1.reshape(S, -1).sum(1).astype(float)
+    return np.allclose(a, b, rtol=0, atol=1e-6 * max(1.0, np.abs(a).max()))
+
+
+@pytest.mark.parametrize("name", ["diff-41", "diff-64", "react-9", "react-64", "chemotaxis"])
+def test_reference_case_matches_oracle(name, tmp_path):
+    out, t, x, final, _ = run_case(name, tmp_path)
+    c = inputs.read_conf(os.path.join(CASES, name, "conf.txt"))
+    assert t == pytest.approx(c["t_end"], abs=1e-6)  # the last step is clamped to endTime
+    model = ffi.Model(c["net"], c["topo"])
+    xo = c["x0"].copy()
+    ffi.run(model, xo, 0.05, 10, SEED, c["t_end"])
+    assert np.array_equal(x.ravel(), xo.astype(np.int64)), name
+    assert conservation_ok(c["net"], c["x0"].astype(np.int64), x.ravel())
+    pops = open(os.path.join(CASES, name, "golden", "pops.txt")).read().split()
+    if name.startswith("diff"):
+        # pure diffusion: totals are an exact invariant, golden pops must match
+        assert [str(v) for v in x.reshape(c["net"]["S"], -1).sum(1)] == pops
+
+
+def test_reference_case_diff5125_mass_and_front(tmp_path):
+    out, t, x, final, _ = run_case("diff-5125", tmp_path)
+    assert int(x.sum()) == 100000  # golden/pops.txt
+    assert (x > 0).sum() > 200  # the front has spread from subvolume 2562
+
+
+def test_log_freq_snapshots_do_not_change_the_trajectory(tmp_path):
+    a = tmp_path / "a"
+    b = tmp_path / "b"
+    a.mkdir()
+    b.mkdir()
+    _, ta, xa, _, _ = run_case("react-9", a)
+    src = os.path.join(CASES, "react-9")
+    for f in os.listdir(src):
+        if f.endswith(".txt"):
+            shutil.copy(os.path.join(src, f), b / f)
+    conf = (b / "conf.txt").read_text().replace("logFreq=0", "logFreq=1")
+    (b / "conf.txt").write_text(conf.rstrip("\n") + f"\nepsilon=0.05\nnc=10\nseed={SEED}\n")
+    r = subprocess.run([DRIVER, "conf.txt"], cwd=b, capture_output=True, text=True, timeout=110)
+    assert r.returncode == 0
+    snaps = glob.glob(str(b / "sim*.dat"))
+    assert len(snaps) >= 8
+    final = max(snaps, key=lambda p: float(re.search(r"_([0-9.]+)\.dat$", p).group(1)))
+    lines = open(final).read().split("\n")
+    xb = np.array([[int(v) for v in lines[4 + i].split()] for i in range(9)], dtype=np.int64).T
+    assert np.array_equal(xa, xb)
