@@ -88,6 +88,10 @@ typedef struct plis_stats {
                                   (the reference's "min(tau) = +Inf" stop,
                                   driver.cu:269-272), <0 error           */
     int32_t pad;
+    uint64_t learned_steps;    /* accepted steps whose tau came from the network */
+    uint64_t learned_rejects;  /* network proposals rejected by the judge  */
+    uint64_t violations;       /* subvolumes over the post-leap bound, summed
+                                  over accepted steps                     */
 } plis_stats;
 
 typedef struct plis_ctx plis_ctx;
@@ -141,6 +145,36 @@ int plis_launch_shape(plis_ctx *ctx, int32_t *out);
  * timing `iters` real iterations per candidate (results do not depend on
  * the launch shape, so the trajectory is unchanged). */
 int plis_autotune(plis_ctx *ctx, uint32_t iters);
+
+/* ---- learned tau proposal (no counterpart in the reference) ----
+ * A 12-64-64-2 MLP (bf16 weights, fp32 accumulation on matrix cores) maps
+ * local features of each subvolume to a log-tau proposal and a critical-flag
+ * logit.  w32 = 5378 fp32 parameters W1[16][64] b1 W2[64][64] b2 W3[64][2]
+ * b3 (rows 12..15 of W1 unused); norm = feature means[12], scales[12], target
+ * mean, target scale (log tau standardisation). */
+int plis_mlp_set(plis_ctx *ctx, const float *w32, const float *norm);
+int plis_mlp_get(plis_ctx *ctx, float *w32);
+/* outputs [n][2] (standardised log tau, critical logit) for feature rows [n][12] */
+int plis_mlp_forward(plis_ctx *ctx, const float *feats, uint32_t n, float *out);
+/* mean loss and its gradient over n (multiple of 64) records, no update */
+int plis_mlp_loss_grad(plis_ctx *ctx, const float *feats, const float *ylog, const float *ycrit,
+                       uint32_t n, float *grad, double *loss);
+/* `steps` Adam steps (lr, beta 0.9/0.999) on random minibatches of `batch`
+ * records drawn from n on the GPU; losses[steps] optional */
+int plis_mlp_train(plis_ctx *ctx, const float *feats, const float *ylog, const float *ycrit,
+                   uint64_t n, uint32_t steps, uint32_t batch, float lr, uint64_t seed,
+                   float *losses);
+/* training records of realization r's current state: features [N][12],
+ * log of Cao's tau' per subvolume [N], critical flag (critical propensity
+ * > 0) [N] */
+int plis_records(plis_ctx *ctx, int32_t r, float *feats, float *ylog, float *crit);
+/* learned = 1: tau = kappa * min_v exp(network log tau) (the critical part
+ * stays exact); a step is kept only if no population goes negative and at
+ * most viol_rate_max * N subvolumes exceed theta * max(eps x / g, 1) in
+ * realized change, otherwise it is redone with Cao's tau.  theta also sets
+ * the bound the violation statistics use in plain mode. */
+int plis_set_tau_mode(plis_ctx *ctx, int32_t learned, double kappa, double theta,
+                      double viol_rate_max);
 
 void plis_destroy(plis_ctx *ctx);
 

@@ -79,16 +79,20 @@ def _declare(L):
 
 def _declare_mlp(L):
     P = C.POINTER
-    L.orc_mlp_forward.argtypes = [P(C.c_float), C.c_int32, P(C.c_float), P(C.c_float)]
-    L.orc_mlp_forward_backward.argtypes = [P(C.c_float), C.c_int32, P(C.c_float),
-                                           P(C.c_float), P(C.c_float), C.c_float,
-                                           P(C.c_float)]
-    L.orc_mlp_forward_backward.restype = C.c_float
-    L.orc_adam_step.argtypes = [P(C.c_float), P(C.c_float), P(C.c_float), P(C.c_float),
-                                C.c_int32, C.c_int32, C.c_float, C.c_float, C.c_float,
-                                C.c_float]
-    L.orc_features.argtypes = [P(Network), P(Topology), P(Params), P(C.c_uint32), P(C.c_float),
-                               C.c_float, P(C.c_float)]
+    L.orc_mlp_param_count.restype = C.c_int
+    L.orc_mlp_forward.argtypes = [P(C.c_float), P(C.c_float), P(C.c_float), C.c_int32, C.c_int32,
+                                  P(C.c_float)]
+    L.orc_mlp_forward.restype = None
+    L.orc_mlp_loss_grad.argtypes = [P(C.c_float), P(C.c_float), P(C.c_float), C.c_int32, C.c_int32,
+                                    P(C.c_float), P(C.c_float), P(C.c_double)]
+    L.orc_mlp_loss_grad.restype = C.c_double
+    L.orc_adam_step.argtypes = [P(C.c_float), P(C.c_float), P(C.c_float), P(C.c_float), C.c_int32,
+                                C.c_int32, C.c_float, C.c_float, C.c_float, C.c_float]
+    L.orc_adam_step.restype = None
+    L.orc_features.argtypes = [P(Network), P(Topology), P(Params), P(C.c_uint32), C.c_float,
+                               P(C.c_float)]
+    L.orc_violations.argtypes = [P(Network), P(Topology), P(Params), P(C.c_uint32),
+                                 P(C.c_uint32), C.c_double, P(C.c_uint8)]
 
 
 def _ptr(a, ct):
@@ -205,3 +209,59 @@ def ssa_batch(model, x0, seed, nreal, t_end):
 
 def num_threads():
     return lib().orc_num_threads()
+
+
+# ---- learned tau reference ----
+def mlp_param_count():
+    return lib().orc_mlp_param_count()
+
+
+def features(model, x, eps, nc, prev_logtau=0.0):
+    x = np.ascontiguousarray(x, dtype=np.uint32)
+    out = np.empty((model.N, 12), dtype=np.float32)
+    p = Params(eps, nc, 0)
+    rc = lib().orc_features(C.byref(model.net), C.byref(model.topo), C.byref(p), _ptr(x, C.c_uint32),
+                            prev_logtau, _ptr(out, C.c_float))
+    assert rc == 0
+    return out
+
+
+def violations(model, x, xn, eps, nc, theta):
+    x = np.ascontiguousarray(x, dtype=np.uint32)
+    xn = np.ascontiguousarray(xn, dtype=np.uint32)
+    flags = np.empty(model.N, dtype=np.uint8)
+    p = Params(eps, nc, 0)
+    n = lib().orc_violations(C.byref(model.net), C.byref(model.topo), C.byref(p), _ptr(x, C.c_uint32),
+                             _ptr(xn, C.c_uint32), theta, _ptr(flags, C.c_uint8))
+    return n, flags
+
+
+def mlp_forward(w, norm, feats, bf16=True):
+    w = np.ascontiguousarray(w, dtype=np.float32)
+    norm = np.ascontiguousarray(norm, dtype=np.float32)
+    feats = np.ascontiguousarray(feats, dtype=np.float32)
+    out = np.empty((feats.shape[0], 2), dtype=np.float32)
+    lib().orc_mlp_forward(_ptr(w, C.c_float), _ptr(norm, C.c_float), _ptr(feats, C.c_float),
+                          feats.shape[0], 1 if bf16 else 0, _ptr(out, C.c_float))
+    return out
+
+
+def mlp_loss_grad(w, norm, feats, ylog, ycrit, bf16=True):
+    w = np.ascontiguousarray(w, dtype=np.float32)
+    norm = np.ascontiguousarray(norm, dtype=np.float32)
+    feats = np.ascontiguousarray(feats, dtype=np.float32)
+    ylog = np.ascontiguousarray(ylog, dtype=np.float32)
+    ycrit = np.ascontiguousarray(ycrit, dtype=np.float32)
+    g = np.empty(mlp_param_count(), dtype=np.float64)
+    loss = lib().orc_mlp_loss_grad(_ptr(w, C.c_float), _ptr(norm, C.c_float), _ptr(feats, C.c_float),
+                                   feats.shape[0], 1 if bf16 else 0, _ptr(ylog, C.c_float),
+                                   _ptr(ycrit, C.c_float), _ptr(g, C.c_double))
+    return loss, g
+
+
+def adam_step(w, m, v, g, t, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
+    for a in (w, m, v):
+        assert a.dtype == np.float32 and a.flags.c_contiguous
+    g = np.ascontiguousarray(g, dtype=np.float32)
+    lib().orc_adam_step(_ptr(w, C.c_float), _ptr(m, C.c_float), _ptr(v, C.c_float), _ptr(g, C.c_float),
+                        w.size, t, lr, b1, b2, eps)

@@ -1,3 +1,148 @@
-/* learned-tau MLP reference: filled in below */
+/*
+ * plis_mlp.c -- CPU reference of the learned tau proposal (TEST
+ * INFRASTRUCTURE ONLY: the checker of the gfx950 MLP kernels).
+ *
+ * Network 12 -> 64 -> 64 -> 2, tanh-form GELU, inputs standardised
+ * per feature.  Parameter blob (float, 5378 values, the first layer padded to
+ * 16 inputs whose last 4 rows stay zero):
+ *   W1[16][64] | b1[64] | W2[64][64] | b2[64] | W3[64][2] | b3[2]
+ * W[i][o] multiplies input i into output o.  With bf16 != 0 every GEMM
+ * operand (weights, activations, back-propagated errors) is rounded to
+ * bfloat16 (round to nearest even) as on the device; sums are in double.
+ *
+ * Loss per sample: (o0 - y)^2 + softplus(o1) - c*o1, averaged over the
+ * batch; y is the standardised log tau target, c in {0,1} the critical flag.
+ *
+ * Features of a subvolume (12, float):
+ *   f[s]     = log(1 + x_s)                         s < 5 (0 if s >= S)
+ *   f[5 + s] = log(1 + sum_{j: r_sj>0} a_j + d_s x_s W_v)
+ *   f[10]    = highest order among reactions with a_j > 0
+ *   f[11]    = log of the realization's previous tau
+ */
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
 #include "plis_oracle.h"
 
+#define NIN 16
+#define NH 64
+#define NOUT 2
+#define P_W1 0
+#define P_B1 (P_W1 + NIN * NH)
+#define P_W2 (P_B1 + NH)
+#define P_B2 (P_W2 + NH * NH)
+#define P_W3 (P_B2 + NH)
+#define P_B3 (P_W3 + NH * NOUT)
+#define P_TOTAL (P_B3 + NOUT)
+
+int orc_mlp_param_count(void) { return P_TOTAL; }
+
+static double bf(double x, int on)
+{
+    if (!on) return x;
+    float f = (float)x;
+    uint32_t b;
+    memcpy(&b, &f, 4);
+    b = (b + 0x7FFFu + ((b >> 16) & 1u)) & 0xFFFF0000u;
+    memcpy(&f, &b, 4);
+    return (double)f;
+}
+
+static const double GC = 0.7978845608028654; /* sqrt(2/pi) */
+/* fp32 rounding of accumulator outputs, only when emulating the device */
+static double f32(double x, int on) { return on ? (double)(float)x : x; }
+static double gelu(double z) { return 0.5 * z * (1.0 + tanh(GC * (z + 0.044715 * z * z * z))); }
+static double gelu_d(double z)
+{
+    double t = tanh(GC * (z + 0.044715 * z * z * z));
+    return 0.5 * (1.0 + t) + 0.5 * z * (1.0 - t * t) * GC * (1.0 + 3.0 * 0.044715 * z * z);
+}
+
+/* norm: mu[12], sig[12]; returns per-sample outputs and (optionally) grads */
+static double mlp_pass(const float *w, const float *norm, const float *feats, int32_t n, int bfon,
+                       const float *ylog, const float *ycrit, double *grad, float *out)
+{
+    double loss = 0.0;
+    for (int32_t i = 0; i < n; i++) {
+        double x[NIN], z1[NH], h1[NH], z2[NH], h2[NH], o[NOUT];
+        for (int k = 0; k < NIN; k++) {
+            double f = k < 12 ? ((double)feats[(size_t)i * 12 + k] - norm[k]) / norm[12 + k] : 0.0;
+            x[k] = bf(f32(f, bfon), bfon);
+        }
+        for (int h = 0; h < NH; h++) {
+            double s = 0.0;
+            for (int k = 0; k < NIN; k++) s += x[k] * bf(w[P_W1 + k * NH + h], bfon);
+            z1[h] = f32(s + w[P_B1 + h], bfon);
+            h1[h] = bf(f32(gelu(z1[h]), bfon), bfon);
+        }
+        for (int h = 0; h < NH; h++) {
+            double s = 0.0;
+            for (int k = 0; k < NH; k++) s += h1[k] * bf(w[P_W2 + k * NH + h], bfon);
+            z2[h] = f32(s + w[P_B2 + h], bfon);
+            h2[h] = bf(f32(gelu(z2[h]), bfon), bfon);
+        }
+        for (int q = 0; q < NOUT; q++) {
+            double s = 0.0;
+            for (int k = 0; k < NH; k++) s += h2[k] * bf(w[P_W3 + k * NOUT + q], bfon);
+            o[q] = f32(s + w[P_B3 + q], bfon);
+        }
+        if (out) { out[(size_t)i * 2] = (float)o[0]; out[(size_t)i * 2 + 1] = (float)o[1]; }
+        if (!grad) continue;
+        double y = ylog[i], c = ycrit[i];
+        double sp = (o[1] > 0 ? o[1] : 0.0) + log1p(exp(-fabs(o[1])));
+        loss += (o[0] - y) * (o[0] - y) + sp - c * o[1];
+        double dO[NOUT], dz2[NH], dz1[NH];
+        dO[0] = bf(f32(2.0 * (o[0] - y) / n, bfon), bfon);
+        dO[1] = bf(f32((1.0 / (1.0 + exp(-o[1])) - c) / n, bfon), bfon);
+        for (int q = 0; q < NOUT; q++) {
+            grad[P_B3 + q] += dO[q];
+            for (int k = 0; k < NH; k++) grad[P_W3 + k * NOUT + q] += h2[k] * dO[q];
+        }
+        for (int k = 0; k < NH; k++) {
+            double s = 0.0;
+            for (int q = 0; q < NOUT; q++) s += dO[q] * bf(w[P_W3 + k * NOUT + q], bfon);
+            dz2[k] = bf(f32(s * gelu_d(z2[k]), bfon), bfon);
+        }
+        for (int h = 0; h < NH; h++) {
+            grad[P_B2 + h] += dz2[h];
+            for (int k = 0; k < NH; k++) grad[P_W2 + k * NH + h] += h1[k] * dz2[h];
+        }
+        for (int k = 0; k < NH; k++) {
+            double s = 0.0;
+            for (int h = 0; h < NH; h++) s += dz2[h] * bf(w[P_W2 + k * NH + h], bfon);
+            dz1[k] = bf(f32(s * gelu_d(z1[k]), bfon), bfon);
+        }
+        for (int h = 0; h < NH; h++) {
+            grad[P_B1 + h] += dz1[h];
+            for (int k = 0; k < NIN; k++) grad[P_W1 + k * NH + h] += x[k] * dz1[h];
+        }
+    }
+    return loss / (n > 0 ? n : 1);
+}
+
+void orc_mlp_forward(const float *w, const float *norm, const float *feats, int32_t n, int32_t bfon,
+                     float *out)
+{
+    mlp_pass(w, norm, feats, n, bfon, NULL, NULL, NULL, out);
+}
+
+double orc_mlp_loss_grad(const float *w, const float *norm, const float *feats, int32_t n,
+                         int32_t bfon, const float *ylog, const float *ycrit, double *grad)
+{
+    memset(grad, 0, sizeof(double) * P_TOTAL);
+    return mlp_pass(w, norm, feats, n, bfon, ylog, ycrit, grad, NULL);
+}
+
+/* Adam (Kingma & Ba 2015) with bias correction; step t counts from 1 */
+void orc_adam_step(float *w, float *m, float *v, const float *g, int32_t count, int32_t t,
+                   float lr, float b1, float b2, float eps)
+{
+    float c1 = 1.0f - powf(b1, (float)t), c2 = 1.0f - powf(b2, (float)t);
+    for (int32_t i = 0; i < count; i++) {
+        m[i] = b1 * m[i] + (1.0f - b1) * g[i];
+        v[i] = b2 * v[i] + (1.0f - b2) * g[i] * g[i];
+        float mh = m[i] / c1, vh = v[i] / c2;
+        w[i] -= lr * mh / (sqrtf(vh) + eps);
+    }
+}

@@ -883,6 +883,71 @@ int orc_ssa_batch(const orc_network *net, const orc_topology *topo, uint64_t see
     return 0;
 }
 
+/* ------------------------------------------------------------------ */
+/* learned-tau inputs: per-subvolume features and post-leap check      */
+/* ------------------------------------------------------------------ */
+int orc_features(const orc_network *net, const orc_topology *topo, const orc_params *p,
+                 const uint32_t *x, float prev_logtau, float *feats)
+{
+    model_t m; csr_t g;
+    if (model_build(net, &m)) { model_free(&m); return -1; }
+    if (csr_build(topo, &g)) { model_free(&m); csr_free(&g); return -1; }
+#pragma omp parallel for schedule(static)
+    for (int64_t vv = 0; vv < (int64_t)g.N; vv++) {
+        uint32_t v = (uint32_t)vv;
+        double a[128];
+        int crit[128];
+        react_props(&m, &g, p, x, v, a, crit);
+        int c = comp_of(&g, v);
+        float *f = feats + (size_t)v * 12;
+        int maxo = 0;
+        for (int j = 0; j < m.R; j++)
+            if (a[j] > 0.0 && m.order[j] > maxo) maxo = m.order[j];
+        for (int s = 0; s < 5; s++) {
+            if (s >= m.S) { f[s] = 0.0f; f[5 + s] = 0.0f; continue; }
+            uint32_t xs = X(x, g.N, s, v);
+            double A = 0.0;
+            for (int j = 0; j < m.R; j++)
+                if (net->reac[s * m.R + j] > 0) A += a[j];
+            A += (m.d[c * m.S + s] * (double)xs) * g.W[v];
+            f[s] = (float)orc_log(1.0 + (double)xs);
+            f[5 + s] = (float)orc_log(1.0 + A);
+        }
+        f[10] = (float)maxo;
+        f[11] = prev_logtau;
+    }
+    model_free(&m); csr_free(&g);
+    return 0;
+}
+
+/* Post-leap leap-condition check (Anderson 2008 form of Cao's bound): the
+ * realized change of every reactant species must stay within
+ * theta * max(eps x / g, 1).  flags[v] = 1 where it does not. */
+int orc_violations(const orc_network *net, const orc_topology *topo, const orc_params *p,
+                   const uint32_t *x, const uint32_t *xn, double theta, uint8_t *flags)
+{
+    model_t m; csr_t g;
+    if (model_build(net, &m)) { model_free(&m); return -1; }
+    if (csr_build(topo, &g)) { model_free(&m); csr_free(&g); return -1; }
+    int64_t count = 0;
+#pragma omp parallel for schedule(static) reduction(+ : count)
+    for (int64_t vv = 0; vv < (int64_t)g.N; vv++) {
+        uint32_t v = (uint32_t)vv;
+        int bad = 0;
+        for (int s = 0; s < m.S; s++) {
+            if (m.hor[s] == 0) continue;
+            uint32_t a = X(x, g.N, s, v), b = X(xn, g.N, s, v);
+            double dx = fabs((double)b - (double)a);
+            double bound = theta * fmax((p->epsilon * (double)a) / g_cao(&m, s, a), 1.0);
+            if (dx > bound) bad = 1;
+        }
+        flags[v] = (uint8_t)bad;
+        count += bad;
+    }
+    model_free(&m); csr_free(&g);
+    return (int)count;
+}
+
 int orc_num_threads(void)
 {
 #ifdef _OPENMP

@@ -98,6 +98,19 @@ int orc_ssa_batch(const orc_network *net, const orc_topology *topo, uint64_t see
 
 int orc_num_threads(void);
 
+/* ---- learned tau (plis_mlp.c) ---- */
+int orc_features(const orc_network *net, const orc_topology *topo, const orc_params *p,
+                 const uint32_t *x, float prev_logtau, float *feats /* N*12 */);
+int orc_violations(const orc_network *net, const orc_topology *topo, const orc_params *p,
+                   const uint32_t *x, const uint32_t *xn, double theta, uint8_t *flags);
+int orc_mlp_param_count(void);
+void orc_mlp_forward(const float *w, const float *norm, const float *feats, int32_t n,
+                     int32_t bfon, float *out);
+double orc_mlp_loss_grad(const float *w, const float *norm, const float *feats, int32_t n,
+                         int32_t bfon, const float *ylog, const float *ycrit, double *grad);
+void orc_adam_step(float *w, float *m, float *v, const float *g, int32_t count, int32_t t,
+                   float lr, float b1, float b2, float eps);
+
 #ifdef __cplusplus
 }
 #endif

@@ -21,7 +21,9 @@ ERRORS = {-1: "PLIS_ERR_ARG", -2: "PLIS_ERR_HIP", -3: "PLIS_ERR_LEAPBACK", -4: "
 EXPORTS = ("plis_create", "plis_run", "plis_run_until", "plis_arm", "plis_iterate", "plis_profile",
            "plis_set_state", "plis_get_state", "plis_get_stats", "plis_sync", "plis_debug_tau",
            "plis_launch_shape", "plis_autotune", "plis_destroy", "plis_last_error",
-           "plis_device_count")
+           "plis_device_count", "plis_mlp_set", "plis_mlp_get", "plis_mlp_forward",
+           "plis_mlp_loss_grad", "plis_mlp_train", "plis_records", "plis_set_tau_mode")
+MLP_PARAMS = 5378
 
 
 class Network(C.Structure):
@@ -44,7 +46,9 @@ class Params(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("t", C.c_double), ("tau", C.c_double), ("steps", C.c_uint64),
                 ("leapbacks", C.c_uint64), ("firings", C.c_uint64), ("subvolume_leaps", C.c_uint64),
-                ("crit_events", C.c_uint64), ("status", C.c_int32), ("pad", C.c_int32)]
+                ("crit_events", C.c_uint64), ("status", C.c_int32), ("pad", C.c_int32),
+                ("learned_steps", C.c_uint64), ("learned_rejects", C.c_uint64),
+                ("violations", C.c_uint64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_ if f != "pad"}
@@ -86,6 +90,14 @@ def _declare(L):
     L.plis_last_error.argtypes = [vp]
     L.plis_last_error.restype = C.c_char_p
     L.plis_device_count.argtypes = [C.c_char_p, C.c_int32]
+    fp = P(C.c_float)
+    L.plis_mlp_set.argtypes = [vp, fp, fp]
+    L.plis_mlp_get.argtypes = [vp, fp]
+    L.plis_mlp_forward.argtypes = [vp, fp, C.c_uint32, fp]
+    L.plis_mlp_loss_grad.argtypes = [vp, fp, fp, fp, C.c_uint32, fp, P(C.c_double)]
+    L.plis_mlp_train.argtypes = [vp, fp, fp, fp, C.c_uint64, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64, fp]
+    L.plis_records.argtypes = [vp, C.c_int32, fp, fp, fp]
+    L.plis_set_tau_mode.argtypes = [vp, C.c_int32, C.c_double, C.c_double, C.c_double]
 
 
 def _ptr(a, ct):
@@ -208,6 +220,58 @@ class Context:
         self._chk(lib().plis_debug_tau(self.h, _ptr(tp, C.c_double), _ptr(a0c, C.c_double),
                                        _ptr(t2, C.c_double)), "plis_debug_tau")
         return tp, a0c, t2
+
+    # ---- learned tau ----
+    def mlp_set(self, w32, norm):
+        w = np.ascontiguousarray(w32, dtype=np.float32)
+        nm = np.ascontiguousarray(norm, dtype=np.float32)
+        assert w.size == MLP_PARAMS and nm.size == 26
+        self._chk(lib().plis_mlp_set(self.h, _ptr(w, C.c_float), _ptr(nm, C.c_float)), "plis_mlp_set")
+
+    def mlp_get(self):
+        w = np.empty(MLP_PARAMS, dtype=np.float32)
+        self._chk(lib().plis_mlp_get(self.h, _ptr(w, C.c_float)), "plis_mlp_get")
+        return w
+
+    def mlp_forward(self, feats):
+        f = np.ascontiguousarray(feats, dtype=np.float32)
+        out = np.empty((f.shape[0], 2), dtype=np.float32)
+        self._chk(lib().plis_mlp_forward(self.h, _ptr(f, C.c_float), f.shape[0], _ptr(out, C.c_float)),
+                  "plis_mlp_forward")
+        return out
+
+    def mlp_loss_grad(self, feats, ylog, ycrit):
+        f = np.ascontiguousarray(feats, dtype=np.float32)
+        y = np.ascontiguousarray(ylog, dtype=np.float32)
+        c = np.ascontiguousarray(ycrit, dtype=np.float32)
+        g = np.empty(MLP_PARAMS, dtype=np.float32)
+        loss = C.c_double()
+        self._chk(lib().plis_mlp_loss_grad(self.h, _ptr(f, C.c_float), _ptr(y, C.c_float), _ptr(c, C.c_float),
+                                           f.shape[0], _ptr(g, C.c_float), C.byref(loss)), "plis_mlp_loss_grad")
+        return loss.value, g
+
+    def mlp_train(self, feats, ylog, ycrit, steps, batch, lr, seed=1):
+        f = np.ascontiguousarray(feats, dtype=np.float32)
+        y = np.ascontiguousarray(ylog, dtype=np.float32)
+        c = np.ascontiguousarray(ycrit, dtype=np.float32)
+        losses = np.empty(steps, dtype=np.float32)
+        self._chk(lib().plis_mlp_train(self.h, _ptr(f, C.c_float), _ptr(y, C.c_float), _ptr(c, C.c_float),
+                                       f.shape[0], steps, batch, lr, seed, _ptr(losses, C.c_float)),
+                  "plis_mlp_train")
+        return losses
+
+    def records(self, r=0):
+        N = self.N[r]
+        f = np.empty((N, 12), dtype=np.float32)
+        y = np.empty(N, dtype=np.float32)
+        c = np.empty(N, dtype=np.float32)
+        self._chk(lib().plis_records(self.h, r, _ptr(f, C.c_float), _ptr(y, C.c_float), _ptr(c, C.c_float)),
+                  "plis_records")
+        return f, y, c
+
+    def set_tau_mode(self, learned, kappa=1.0, theta=3.0, viol_rate_max=1.0):
+        self._chk(lib().plis_set_tau_mode(self.h, 1 if learned else 0, kappa, theta, viol_rate_max),
+                  "plis_set_tau_mode")
 
     def close(self):
         if getattr(self, "h", None):

@@ -16,7 +16,7 @@
 #include <vector>
 
 #include "../../include/plis.h"
-#include "plis_kernels.hip"
+#include "plis_learned.hip"
 
 using namespace plis;
 
@@ -57,6 +57,16 @@ struct plis_ctx {
     hipGraphExec_t graph = nullptr;
     uint32_t graph_iters = 0;
     std::vector<void*> allocs;
+    // post-leap check and learned tau
+    double theta = 3.0, viol_rate_max = 1.0, kappa = 1.0;
+    int learned_on = 0;
+    float *d_w32 = nullptr, *d_m = nullptr, *d_v = nullptr, *d_norm = nullptr, *d_grad = nullptr;
+    __bf16* d_q = nullptr;
+    float* d_partial = nullptr;
+    uint32_t train_blocks = 0;
+    int32_t adam_t = 0;
+    float *d_rf = nullptr, *d_ry = nullptr, *d_rc = nullptr;  // training records on the device
+    uint64_t rec_cap = 0;
 };
 
 #define HIPCHK(ctx, call)                                                              \
@@ -263,36 +273,62 @@ static KernelArgs make_args(plis_ctx* c) {
     a.seed_lo = (uint32_t)c->seed;
     a.seed_hi = (uint32_t)(c->seed >> 32);
     a.waves_per_block = c->wpb;
+    a.theta = c->theta;
+    a.viol_rate_max = c->viol_rate_max;
+    a.kappa = c->kappa;
+    a.learned_on = c->learned_on;
+    a.mlp_w32 = c->d_w32;
+    a.mlp_q = c->d_q;
+    a.mlp_norm = c->d_norm;
     return a;
 }
 
 static const int kVariantS[] = {1, 3, 5, 8, 64};
 static const int kVariantR[] = {4, 4, 8, 16, 128};
 
-// Kernels of one step: 0 tau pass, 1 decide, 2 leap pass, 3 commit.
-static void launch_kernel(plis_ctx* c, int which, const KernelArgs& a) {
+// Kernels of one step: 0 tau pass (learned and/or Cao), 1 decide, 2 leap
+// pass, 3 commit; 5 = training records of one realization.
+template <int MS, int MR>
+static void launch_variant(plis_ctx* c, int which, const KernelArgs& a) {
     const dim3 grid(c->blocks), block(64 * c->wpb), rgrid(c->nreal), rblock(256);
-    if (which == 3) {
-        hipLaunchKernelGGL(commit_kernel, rgrid, rblock, 0, c->stream, a);
-        return;
-    }
-#define PLIS_LAUNCH(V, MS, MR)                                                                        \
-    case V:                                                                                            \
-        if (which == 0) hipLaunchKernelGGL((phase_a_kernel<MS, MR>), grid, block, 0, c->stream, a);     \
-        else if (which == 1) hipLaunchKernelGGL((decide_kernel<MS, MR>), rgrid, rblock, 0, c->stream, a); \
-        else if (c->edge_parallel && MS <= 8)                                                          \
-            hipLaunchKernelGGL((phase_b_kernel<MS, MR, (MS <= 8)>), grid, block, 0, c->stream, a);      \
-        else hipLaunchKernelGGL((phase_b_kernel<MS, MR, false>), grid, block, 0, c->stream, a);         \
-        break;
-    switch (c->variant) {
-        PLIS_LAUNCH(0, 1, 4)
-        PLIS_LAUNCH(1, 3, 4)
-        PLIS_LAUNCH(2, 5, 8)
-        PLIS_LAUNCH(3, 8, 16)
+    switch (which) {
+        case 0:
+            if constexpr (MS <= 8) {
+                if (c->learned_on) hipLaunchKernelGGL((learned_tau_kernel<MS, MR>), grid, block, 0, c->stream, a);
+            }
+            hipLaunchKernelGGL((phase_a_kernel<MS, MR>), grid, block, 0, c->stream, a);
+            break;
+        case 1:
+            hipLaunchKernelGGL((decide_kernel<MS, MR>), rgrid, rblock, 0, c->stream, a);
+            break;
+        case 2:
+            if constexpr (MS <= 8) {
+                if (c->edge_parallel) {
+                    hipLaunchKernelGGL((phase_b_kernel<MS, MR, true>), grid, block, 0, c->stream, a);
+                    break;
+                }
+            }
+            hipLaunchKernelGGL((phase_b_kernel<MS, MR, false>), grid, block, 0, c->stream, a);
+            break;
+        case 3:
+            hipLaunchKernelGGL(commit_kernel, rgrid, rblock, 0, c->stream, a);
+            break;
+        case 5:
+            if constexpr (MS <= 8) hipLaunchKernelGGL((records_kernel<MS, MR>), grid, block, 0, c->stream, a);
+            break;
         default:
-        PLIS_LAUNCH(4, 64, 128)
+            break;
     }
-#undef PLIS_LAUNCH
+}
+
+static void launch_kernel(plis_ctx* c, int which, const KernelArgs& a) {
+    switch (c->variant) {
+        case 0: launch_variant<1, 4>(c, which, a); break;
+        case 1: launch_variant<3, 4>(c, which, a); break;
+        case 2: launch_variant<5, 8>(c, which, a); break;
+        case 3: launch_variant<8, 16>(c, which, a); break;
+        default: launch_variant<64, 128>(c, which, a); break;
+    }
 }
 
 static void launch_step(plis_ctx* c, const KernelArgs& a) {
@@ -684,6 +720,9 @@ extern "C" int plis_get_stats(plis_ctx* c, int32_t r, plis_stats* st) {
     st->crit_events = q.crit_events;
     st->status = q.status;
     st->pad = 0;
+    st->learned_steps = q.learned_steps;
+    st->learned_rejects = q.learned_rejects;
+    st->violations = q.violations;
     return PLIS_OK;
 }
 
@@ -698,7 +737,11 @@ extern "C" int plis_debug_tau(plis_ctx* c, double* tp, double* a0c, double* t2) 
     a.dbg_tp = d_tp;
     a.dbg_a0c = d_a0c;
     a.dbg_t2 = d_t2;
+    a.learned_on = 0;
+    const int lo = c->learned_on;
+    c->learned_on = 0;
     launch_kernel(c, 0, a);
+    c->learned_on = lo;
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e == hipSuccess) e = hipMemcpy(tp, d_tp, n * sizeof(double), hipMemcpyDeviceToHost);
@@ -742,5 +785,241 @@ extern "C" int plis_autotune(plis_ctx* c, uint32_t iters) {
             }
         }
     set_shape(c, bw, bb);
+    return PLIS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// learned tau: network state, training on the GPU, records, step mode
+// ---------------------------------------------------------------------------
+static int mlp_alloc(plis_ctx* c) {
+    if (c->d_w32) return 0;
+    int rc;
+    std::vector<float> zero(kPTotal + 1, 0.0f), norm(26, 0.0f);
+    for (int k = 12; k < 24; ++k) norm[k] = 1.0f;
+    norm[25] = 1.0f;
+    c->train_blocks = 256;
+    if ((rc = upload(c, &c->d_w32, zero)) || (rc = upload(c, &c->d_m, zero)) || (rc = upload(c, &c->d_v, zero)) ||
+        (rc = upload(c, &c->d_grad, zero)) || (rc = upload(c, &c->d_norm, norm)) ||
+        (rc = dev_alloc(c, &c->d_q, (size_t)kQTotal)) ||
+        (rc = dev_alloc(c, &c->d_partial, (size_t)c->train_blocks * (kPTotal + 1))))
+        return rc;
+    hipLaunchKernelGGL(mlp_pack_kernel, dim3((kQTotal + 255) / 256), dim3(256), 0, c->stream, c->d_w32, c->d_q);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
+extern "C" int plis_mlp_set(plis_ctx* c, const float* w32, const float* norm) {
+    if (!c || !w32 || !norm) return PLIS_ERR_ARG;
+    if (int rc = mlp_alloc(c)) return rc;
+    for (int k = 0; k < 12; ++k)
+        if (!(norm[12 + k] > 0.0f)) {
+            c->err = "feature scales must be > 0";
+            return PLIS_ERR_ARG;
+        }
+    if (!(norm[25] > 0.0f)) {
+        c->err = "target scale must be > 0";
+        return PLIS_ERR_ARG;
+    }
+    std::vector<float> w(w32, w32 + kPTotal);
+    for (int i = 12 * kNh; i < kNin * kNh; ++i) w[i] = 0.0f;  // padded inputs
+    std::vector<float> zero(kPTotal, 0.0f);
+    HIPCHK(c, hipMemcpyAsync(c->d_w32, w.data(), kPTotal * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_m, zero.data(), kPTotal * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_v, zero.data(), kPTotal * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_norm, norm, 26 * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(mlp_pack_kernel, dim3((kQTotal + 255) / 256), dim3(256), 0, c->stream, c->d_w32, c->d_q);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->adam_t = 0;
+    return PLIS_OK;
+}
+
+extern "C" int plis_mlp_get(plis_ctx* c, float* w32) {
+    if (!c || !w32) return PLIS_ERR_ARG;
+    if (!c->d_w32) {
+        c->err = "no network set";
+        return PLIS_ERR_STATE;
+    }
+    HIPCHK(c, hipMemcpyAsync(w32, c->d_w32, kPTotal * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PLIS_OK;
+}
+
+extern "C" int plis_mlp_forward(plis_ctx* c, const float* feats, uint32_t n, float* out) {
+    if (!c || !feats || !out || n == 0) return PLIS_ERR_ARG;
+    if (!c->d_w32) {
+        c->err = "no network set";
+        return PLIS_ERR_STATE;
+    }
+    float *d_f = nullptr, *d_o = nullptr;
+    HIPCHK(c, hipMalloc((void**)&d_f, (size_t)n * 12 * sizeof(float)));
+    hipError_t e = hipMalloc((void**)&d_o, (size_t)n * 2 * sizeof(float));
+    if (e == hipSuccess) e = hipMemcpy(d_f, feats, (size_t)n * 12 * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        const MlpDev m{c->d_w32, c->d_q, c->d_norm};
+        const uint32_t tiles = (n + 63) / 64;
+        hipLaunchKernelGGL(mlp_infer_kernel, dim3(std::min<uint32_t>(tiles, 4096)), dim3(64), 0, c->stream, m, d_f,
+                           n, d_o);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipMemcpy(out, d_o, (size_t)n * 2 * sizeof(float), hipMemcpyDeviceToHost);
+    (void)hipFree(d_f);
+    if (d_o) (void)hipFree(d_o);
+    if (e != hipSuccess) {
+        c->err = std::string("plis_mlp_forward: ") + hipGetErrorString(e);
+        return PLIS_ERR_HIP;
+    }
+    return PLIS_OK;
+}
+
+static int upload_records(plis_ctx* c, const float* feats, const float* ylog, const float* ycrit, uint64_t n) {
+    if (n > c->rec_cap) {
+        for (float* p : {c->d_rf, c->d_ry, c->d_rc})
+            if (p) {
+                c->allocs.erase(std::remove(c->allocs.begin(), c->allocs.end(), (void*)p), c->allocs.end());
+                (void)hipFree(p);
+            }
+        c->d_rf = c->d_ry = c->d_rc = nullptr;
+        int rc;
+        if ((rc = dev_alloc(c, &c->d_rf, n * 12)) || (rc = dev_alloc(c, &c->d_ry, n)) ||
+            (rc = dev_alloc(c, &c->d_rc, n)))
+            return rc;
+        c->rec_cap = n;
+    }
+    HIPCHK(c, hipMemcpyAsync(c->d_rf, feats, n * 12 * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_ry, ylog, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_rc, ycrit, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    return 0;
+}
+
+static void train_launch(plis_ctx* c, uint64_t n, uint32_t batch, uint32_t step, uint64_t seed, int full,
+                         int32_t adam_t, float lr) {
+    TrainArgs ta;
+    ta.m = MlpDev{c->d_w32, c->d_q, c->d_norm};
+    ta.feats = c->d_rf;
+    ta.ylog = c->d_ry;
+    ta.ycrit = c->d_rc;
+    ta.n = n;
+    ta.batch = batch;
+    ta.step = step;
+    ta.seed_lo = (uint32_t)seed;
+    ta.seed_hi = (uint32_t)(seed >> 32);
+    ta.full_pass = full;
+    ta.partial = c->d_partial;
+    const uint32_t blocks = std::min<uint32_t>(c->train_blocks, batch / 64);
+    hipLaunchKernelGGL(mlp_train_kernel, dim3(blocks), dim3(64), 0, c->stream, ta);
+    AdamArgs aa{c->d_w32, c->d_m, c->d_v, c->d_grad, c->d_partial, blocks, adam_t, lr, 0.9f, 0.999f, 1e-8f};
+    hipLaunchKernelGGL(mlp_adam_kernel, dim3((kPTotal + 1 + 255) / 256), dim3(256), 0, c->stream, aa);
+    if (adam_t > 0)
+        hipLaunchKernelGGL(mlp_pack_kernel, dim3((kQTotal + 255) / 256), dim3(256), 0, c->stream, c->d_w32, c->d_q);
+}
+
+extern "C" int plis_mlp_loss_grad(plis_ctx* c, const float* feats, const float* ylog, const float* ycrit, uint32_t n,
+                                  float* grad, double* loss) {
+    if (!c || !feats || !ylog || !ycrit || !grad || n == 0 || n % 64 != 0) {
+        if (c) c->err = "n must be a positive multiple of 64";
+        return PLIS_ERR_ARG;
+    }
+    if (!c->d_w32) {
+        c->err = "no network set";
+        return PLIS_ERR_STATE;
+    }
+    if (int rc = upload_records(c, feats, ylog, ycrit, n)) return rc;
+    train_launch(c, n, n, 0, 0, 1, 0, 0.0f);
+    HIPCHK(c, hipGetLastError());
+    std::vector<float> g(kPTotal + 1);
+    HIPCHK(c, hipMemcpyAsync(g.data(), c->d_grad, (kPTotal + 1) * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::copy(g.begin(), g.begin() + kPTotal, grad);
+    if (loss) *loss = (double)g[kPTotal] / (double)n;
+    return PLIS_OK;
+}
+
+extern "C" int plis_mlp_train(plis_ctx* c, const float* feats, const float* ylog, const float* ycrit, uint64_t n,
+                              uint32_t steps, uint32_t batch, float lr, uint64_t seed, float* losses) {
+    if (!c || !feats || !ylog || !ycrit || n == 0 || batch == 0 || batch % 64 != 0) {
+        if (c) c->err = "batch must be a positive multiple of 64";
+        return PLIS_ERR_ARG;
+    }
+    if (!c->d_w32) {
+        c->err = "no network set";
+        return PLIS_ERR_STATE;
+    }
+    if (int rc = upload_records(c, feats, ylog, ycrit, n)) return rc;
+    float* d_losses = nullptr;
+    if (losses) HIPCHK(c, hipMalloc((void**)&d_losses, (size_t)steps * sizeof(float)));
+    for (uint32_t t = 0; t < steps; ++t) {
+        train_launch(c, n, batch, t, seed, 0, ++c->adam_t, lr);
+        if (d_losses)
+            (void)hipMemcpyAsync(d_losses + t, c->d_grad + kPTotal, sizeof(float), hipMemcpyDeviceToDevice, c->stream);
+    }
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess && losses) {
+        e = hipMemcpy(losses, d_losses, (size_t)steps * sizeof(float), hipMemcpyDeviceToHost);
+        for (uint32_t t = 0; t < steps; ++t) losses[t] /= (float)batch;
+    }
+    if (d_losses) (void)hipFree(d_losses);
+    if (e != hipSuccess) {
+        c->err = std::string("plis_mlp_train: ") + hipGetErrorString(e);
+        return PLIS_ERR_HIP;
+    }
+    return PLIS_OK;
+}
+
+extern "C" int plis_records(plis_ctx* c, int32_t r, float* feats, float* ylog, float* crit) {
+    if (!c || !feats || !ylog || !crit || r < 0 || (uint32_t)r >= c->nreal) return PLIS_ERR_ARG;
+    if (c->variant > 3) {
+        c->err = "learned tau supports networks of at most 8 species and 16 reactions";
+        return PLIS_ERR_ARG;
+    }
+    const uint32_t N = c->info[r].N;
+    float *df, *dy, *dc;
+    HIPCHK(c, hipMalloc((void**)&df, (size_t)N * 12 * sizeof(float)));
+    HIPCHK(c, hipMalloc((void**)&dy, (size_t)N * sizeof(float)));
+    HIPCHK(c, hipMalloc((void**)&dc, (size_t)N * sizeof(float)));
+    KernelArgs a = make_args(c);
+    a.rec_feats = df;
+    a.rec_ylog = dy;
+    a.rec_crit = dc;
+    a.rec_real = (uint32_t)r;
+    launch_kernel(c, 5, a);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipMemcpy(feats, df, (size_t)N * 12 * sizeof(float), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(ylog, dy, (size_t)N * sizeof(float), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(crit, dc, (size_t)N * sizeof(float), hipMemcpyDeviceToHost);
+    (void)hipFree(df);
+    (void)hipFree(dy);
+    (void)hipFree(dc);
+    if (e != hipSuccess) {
+        c->err = std::string("plis_records: ") + hipGetErrorString(e);
+        return PLIS_ERR_HIP;
+    }
+    return PLIS_OK;
+}
+
+extern "C" int plis_set_tau_mode(plis_ctx* c, int32_t learned, double kappa, double theta, double viol_rate_max) {
+    if (!c || !(kappa > 0.0) || !(theta > 0.0) || !(viol_rate_max >= 0.0)) return PLIS_ERR_ARG;
+    if (learned && (c->variant > 3 || !c->d_w32)) {
+        c->err = c->d_w32 ? "learned tau supports networks of at most 8 species and 16 reactions"
+                          : "set the network (plis_mlp_set) before enabling learned tau";
+        return PLIS_ERR_STATE;
+    }
+    c->learned_on = learned ? 1 : 0;
+    c->kappa = kappa;
+    c->theta = theta;
+    c->viol_rate_max = viol_rate_max;
+    if (c->graph) {
+        (void)hipGraphExecDestroy(c->graph);
+        c->graph = nullptr;
+        c->graph_iters = 0;
+    }
+    if (int rc = pull_ctl(c)) return rc;
+    for (auto& q : c->ctl_host) q.fallback = 0u;
+    HIPCHK(c, hipMemcpyAsync(c->d_ctl, c->ctl_host.data(), c->ctl_host.size() * sizeof(RealCtl),
+                             hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return PLIS_OK;
 }

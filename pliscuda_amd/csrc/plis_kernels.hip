@@ -92,6 +92,23 @@ __device__ __forceinline__ double g_cao(int hor, int mult, uint32_t xs) {
     return 3.0;
 }
 
+// Post-leap check (Anderson 2008 form of Cao's leap condition): a reactant
+// species whose realized change exceeds theta * max(eps x / g, 1).
+template <int MS>
+__device__ __forceinline__ uint32_t leap_violation(const int32_t* hor, const int32_t* mult, int S,
+                                                   const uint32_t (&x)[MS], const long long (&nx)[MS],
+                                                   double eps, double theta) {
+    uint32_t bad = 0;
+#pragma unroll
+    for (int s = 0; s < MS; ++s) {
+        if (s >= S || hor[s] == 0) continue;
+        const double dx = __builtin_fabs((double)nx[s] - (double)x[s]);
+        const double bound = theta * fmax((eps * (double)x[s]) / g_cao(hor[s], mult[s], x[s]), 1.0);
+        if (dx > bound) bad = 1u;
+    }
+    return bad;
+}
+
 // Reaction propensities a_j = k_j prod_s C(x_s, r_sj) and the critical flags
 // (Cao: L_j = min_s floor(x_s / r_sj) < n_c) at one subvolume.
 template <int MR, int MS>
@@ -336,7 +353,7 @@ __device__ void finalize_A(const KernelArgs& a, const NetView& n, const RealInfo
     c.tau_p = tp;
     c.tau2 = t2;
     c.vstar = v2;
-    c.attempt = 0u;
+    c.learned = (a.learned_on && !c.fallback) ? 1u : 0u;
     c.ev_kind = 0;
     c.ev_idx = -1;
     c.ev_dst = v2;
@@ -358,6 +375,7 @@ __global__ __launch_bounds__(256) void phase_a_kernel(KernelArgs a) {
         const uint32_t done = wave_uniform(a.ctl[r].done), retry = wave_uniform(a.ctl[r].retry);
         const bool dbg = a.dbg_tp != nullptr;
         if (!dbg && (done || retry)) continue;
+        if (!dbg && a.learned_on && !wave_uniform(a.ctl[r].fallback)) continue;  // learned pass
         const uint32_t cur = wave_uniform(a.ctl[r].cur);
         const uint32_t* X = cur ? a.x1 : a.x0;
         const uint32_t v = te.v0 + lane;
@@ -448,7 +466,8 @@ struct StepView {
 template <int MS, int MR>
 __device__ __forceinline__ bool leap_subvolume(const KernelArgs& a, const NetView& n, const RealInfo& ri,
                                                const uint32_t* X, uint32_t* Xn, uint32_t v,
-                                               const StepView& sv, unsigned long long& fire_out) {
+                                               const StepView& sv, unsigned long long& fire_out,
+                                               uint32_t& viol) {
     const uint32_t N = ri.N;
     const int nc = a.n_critical;
     uint32_t x[MS];
@@ -618,6 +637,7 @@ __device__ __forceinline__ bool leap_subvolume(const KernelArgs& a, const NetVie
         Xn[ri.x_off + (uint64_t)s * N + v] = (uint32_t)nx[s];
     }
     fire_out = fire;
+    viol = leap_violation<MS>(n.hor, n.mult, n.S, x, nx, a.epsilon, a.theta);
     return bad;
 }
 
@@ -627,7 +647,7 @@ template <int MS, int MR>
 __device__ __forceinline__ bool finish_subvolume(const KernelArgs& a, const NetView& n, const RealInfo& ri,
                                                  uint32_t* Xn, uint32_t v, const uint32_t (&x)[MS], int c,
                                                  const long long (&hop)[MS], const StepView& sv,
-                                                 unsigned long long& fire) {
+                                                 unsigned long long& fire, uint32_t& viol) {
     const uint32_t N = ri.N;
     double av[MR];
     uint32_t crit[(MR + 31) / 32];
@@ -681,6 +701,7 @@ __device__ __forceinline__ bool finish_subvolume(const KernelArgs& a, const NetV
         bad = bad || nx[s] < 0 || nx[s] > 0xFFFFFFFFll;
         Xn[ri.x_off + (uint64_t)s * N + v] = (uint32_t)nx[s];
     }
+    viol = leap_violation<MS>(n.hor, n.mult, n.S, x, nx, a.epsilon, a.theta);
     return bad;
 }
 
@@ -696,7 +717,8 @@ __device__ __forceinline__ void leap_tile_edges(const KernelArgs& a, const NetVi
                                                 const uint32_t* X, uint32_t* Xn, uint32_t v0,
                                                 const StepView& sv, uint32_t (*s_x)[kTile],
                                                 unsigned long long (*s_del)[kTile], uint32_t* s_rp,
-                                                int32_t* s_c, unsigned long long& fire, uint32_t& bad) {
+                                                int32_t* s_c, unsigned long long& fire, uint32_t& bad,
+                                                uint32_t& viol) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t N = ri.N;
     const int nc = a.n_critical;
@@ -782,24 +804,37 @@ __device__ __forceinline__ void leap_tile_edges(const KernelArgs& a, const NetVi
         long long hop[MS];
 #pragma unroll
         for (int s = 0; s < MS; ++s) hop[s] = (long long)s_del[s][lane];
-        bad = finish_subvolume<MS, MR>(a, n, ri, Xn, v, x, c, hop, sv, fire) ? 1u : 0u;
+        bad = finish_subvolume<MS, MR>(a, n, ri, Xn, v, x, c, hop, sv, fire, viol) ? 1u : 0u;
     }
 }
 
-__device__ __forceinline__ void reduce_partB(unsigned long long& fire, uint32_t& bad) {
+__device__ __forceinline__ void reduce_partB(unsigned long long& fire, uint32_t& bad, uint32_t& viol) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         fire += __shfl_xor(fire, off);
         bad |= (uint32_t)__shfl_xor((int)bad, off);
+        viol += (uint32_t)__shfl_xor((int)viol, off);
     }
 }
 
 __device__ void finalize_B(const KernelArgs& a, const RealInfo& ri, uint32_t r, unsigned long long fire,
-                           uint32_t bad) {
+                           uint32_t bad, uint32_t viol) {
     RealCtl c = a.ctl[r];
-    if (bad) {
-        c.leapbacks += 1;
+    // A learned proposal is judged by Cao's leap condition after the fact: it
+    // is kept only without negative populations and with no more violating
+    // subvolumes than the plain method produces (viol_rate_max); otherwise
+    // the step is redone with Cao's tau and fresh draws.
+    const bool learned_reject =
+        c.learned && (bad || (double)viol > a.viol_rate_max * (double)ri.N);
+    if (bad || learned_reject) {
         c.attempt += 1u;
+        if (learned_reject) {
+            c.learned_rejects += 1;
+            c.fallback = 1u;
+            c.retry = 0u;
+        } else {
+            c.leapbacks += 1;
+        }
         if (c.attempt > kMaxAttempt) {
             c.status = kErrLeapback;
             c.done = 1u;
@@ -807,9 +842,11 @@ __device__ void finalize_B(const KernelArgs& a, const RealInfo& ri, uint32_t r, 
             atomicAdd(a.ndone, 1u);
             return;
         }
-        c.tau_p = c.tau_p * 0.5;
-        decide(c);
-        c.retry = 1u;
+        if (!learned_reject) {
+            c.tau_p = c.tau_p * 0.5;
+            decide(c);
+            c.retry = 1u;
+        }
         a.ctl[r] = c;
         return;
     }
@@ -819,7 +856,10 @@ __device__ void finalize_B(const KernelArgs& a, const RealInfo& ri, uint32_t r, 
     c.steps += 1;
     c.firings += fire;
     c.subvolume_leaps += ri.N;
+    c.violations += viol;
     if (c.crit) c.crit_events += 1;
+    if (c.learned) c.learned_steps += 1;
+    c.fallback = 0u;
     c.attempt = 0u;
     c.retry = 0u;
     if (c.t >= c.t_end || c.steps >= c.step_limit || c.t >= c.t_pause) {
@@ -867,14 +907,15 @@ __global__ __launch_bounds__(256) void phase_b_kernel(KernelArgs a) {
         uint32_t* Xn = cur ? a.x0 : a.x1;
         const uint32_t v = te.v0 + lane;
         unsigned long long fire = 0;
-        uint32_t bad = 0;
+        uint32_t bad = 0, viol = 0;
         if constexpr (kEdgePar) {
-            leap_tile_edges<MS, MR>(a, n, ri, X, Xn, te.v0, sv, s_x[wv], s_del[wv], s_rp[wv], s_c[wv], fire, bad);
+            leap_tile_edges<MS, MR>(a, n, ri, X, Xn, te.v0, sv, s_x[wv], s_del[wv], s_rp[wv], s_c[wv], fire, bad,
+                                    viol);
         } else {
-            if (v < ri.N) bad = leap_subvolume<MS, MR>(a, n, ri, X, Xn, v, sv, fire) ? 1u : 0u;
+            if (v < ri.N) bad = leap_subvolume<MS, MR>(a, n, ri, X, Xn, v, sv, fire, viol) ? 1u : 0u;
         }
-        reduce_partB(fire, bad);
-        if (lane == 0) a.partB_tile[t] = PartB{fire, bad, 0u};
+        reduce_partB(fire, bad, viol);
+        if (lane == 0) a.partB_tile[t] = PartB{fire, bad, viol};
     }
 }
 
@@ -886,9 +927,9 @@ __global__ __launch_bounds__(256) void commit_kernel(KernelArgs a) {
     const RealInfo ri = a.info[r];
     if (a.ctl[r].done) return;
     __shared__ unsigned long long s_f[4];
-    __shared__ uint32_t s_b[4];
+    __shared__ uint32_t s_b[4], s_v[4];
     unsigned long long fire = 0;
-    uint32_t bad = 0;
+    uint32_t bad = 0, viol = 0;
     for (uint32_t q0 = 0; q0 < ri.ntiles; q0 += 8u * blockDim.x) {
         PartB p[8];
 #pragma unroll
@@ -900,21 +941,24 @@ __global__ __launch_bounds__(256) void commit_kernel(KernelArgs a) {
         for (int i = 0; i < 8; ++i) {
             fire += p[i].fire;
             bad |= p[i].bad;
+            viol += p[i].viol;
         }
     }
-    reduce_partB(fire, bad);
+    reduce_partB(fire, bad, viol);
     const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     if (lane == 0) {
         s_f[wv] = fire;
         s_b[wv] = bad;
+        s_v[wv] = viol;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         for (uint32_t w = 1; w < (blockDim.x >> 6); ++w) {
             fire += s_f[w];
             bad |= s_b[w];
+            viol += s_v[w];
         }
-        finalize_B(a, ri, r, fire, bad);
+        finalize_B(a, ri, r, fire, bad, viol);
     }
 }
 

@@ -60,6 +60,10 @@ struct RealCtl {
     int32_t crit, last, ev_kind, ev_idx;
     uint32_t done, retry, cur;
     int32_t status;
+    // learned tau: this step's tau came from the network / the step after a
+    // rejected proposal uses Cao's formula
+    uint32_t learned, fallback;
+    uint64_t learned_steps, learned_rejects, violations;
 };
 
 struct TileEnt {
@@ -73,7 +77,7 @@ struct PartA {
 };
 struct PartB {
     unsigned long long fire;
-    uint32_t bad, pad;
+    uint32_t bad, viol;
 };
 
 struct KernelArgs {
@@ -111,6 +115,19 @@ struct KernelArgs {
     double* dbg_a0c;
     double* dbg_t2;
     uint32_t waves_per_block;
+    // post-leap check |x' - x| <= theta max(eps x / g, 1) and learned tau
+    double theta;
+    double viol_rate_max;    // accepted fraction of violating subvolumes (learned steps)
+    double kappa;            // multiplier of the learned proposal
+    int32_t learned_on;
+    const float* mlp_w32;    // fp32 master blob
+    const void* mlp_q;       // packed bf16 copies
+    const float* mlp_norm;   // feature / target standardisation
+    // records output (records kernel)
+    float* rec_feats;
+    float* rec_ylog;
+    float* rec_crit;
+    uint32_t rec_real;
 };
 
 }  // namespace plis
