@@ -92,6 +92,56 @@ def gpu_measure(wl, steps, warmup, seed=2024, autotune=True):
             "commit_ms": prof[4] / n_prof, "iter_ms_events": prof[0] / n_prof}
 
 
+def learned_measure(wl, steps, warmup, train_steps, batch, seed=2024, kappa=1.0):
+    """Records from plain steps of the same workload, GPU training, then the
+    learned step timed like the plain one."""
+    import torch
+
+    from pliscuda_amd import capi, learned
+    N = int(sum(t["N"] for t in wl["topos"]))
+    rec = capi.Context(wl["net"], wl["topos"], wl["x0"], eps=wl["eps"], nc=wl["nc"], seed=seed + 1)
+    t0 = time.perf_counter()
+    F, Y, Cc = learned.collect_records(rec, snapshots=16, stride=4, per_snapshot=16384)
+    t_rec = time.perf_counter() - t0
+    plain_stats = [rec.stats(r) for r in range(rec.n)]
+    vmax = learned.calibrate_viol_rate(plain_stats, N)
+    t0 = time.perf_counter()
+    norm, losses = learned.train(rec, F, Y, Cc, steps=train_steps, batch=batch, lr=1e-3)
+    rec.sync()
+    t_train = time.perf_counter() - t0
+    w = rec.mlp_get()
+    rec.close()
+    ctx = capi.Context(wl["net"], wl["topos"], wl["x0"], eps=wl["eps"], nc=wl["nc"], seed=seed)
+    ctx.mlp_set(w, norm)
+    ctx.set_tau_mode(True, kappa=kappa, theta=3.0, viol_rate_max=vmax)
+    ctx.arm(t_end=float("inf"))
+    ctx.iterate(warmup)
+    ctx.sync()
+    s0 = [ctx.stats(r) for r in range(ctx.n)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ctx.iterate(steps)
+    ctx.sync()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    s1 = [ctx.stats(r) for r in range(ctx.n)]
+    prof = ctx.profile(min(steps, 20))
+    ctx.close()
+    d = lambda f: sum(b[f] - a[f] for a, b in zip(s0, s1))
+    props = d("learned_steps") + d("learned_rejects")
+    return {"seconds": dt, "subvolume_leaps": d("subvolume_leaps"), "firings": d("firings"),
+            "accepted_steps": d("steps"), "learned_steps": d("learned_steps"),
+            "learned_rejects": d("learned_rejects"), "leapbacks": d("leapbacks"),
+            "rejection_rate": d("learned_rejects") / props if props else None,
+            "subvolume_simtime": sum((b["t"] - a["t"]) * n for a, b, n in
+                                     zip(s0, s1, [int(t["N"]) for t in wl["topos"]])),
+            "tau_ms": prof[1] / min(steps, 20), "leap_ms": prof[3] / min(steps, 20),
+            "records": int(F.shape[0]), "records_seconds": t_rec, "train_steps": train_steps,
+            "train_batch": batch, "train_seconds": t_train,
+            "loss_first": float(losses[:10].mean()), "loss_last": float(losses[-100:].mean()),
+            "viol_rate_max": vmax, "kappa": kappa}
+
+
 def cpu_baseline(wl, seed, sample_steps, gpu_check=True):
     """Oracle on the host cores for sample_steps accepted steps (bounded)."""
     from oracle import ffi
@@ -156,6 +206,10 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--extra", action="store_true", help="also measure the Schlogl grid and batch")
     ap.add_argument("--copy-probe", action="store_true", help="only run the counter-calibration copy")
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--train-steps", type=int, default=20000)
+    ap.add_argument("--train-batch", type=int, default=65536)
+    ap.add_argument("--kappa", type=float, default=1.0)
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
@@ -210,6 +264,18 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic},
     }
+    if not args.no_learned and args.workload == "brusselator":
+        lr_ = learned_measure(wl, args.steps, args.warmup, args.train_steps, args.train_batch,
+                              kappa=args.kappa)
+        lv = lr_["subvolume_leaps"] / lr_["seconds"]
+        simt_plain = res["subvolume_simtime"] / res["seconds"]
+        simt_learned = lr_["subvolume_simtime"] / lr_["seconds"]
+        out["learned"] = {"subvolume_leaps_per_s": lv, "firings_per_s": lr_["firings"] / lr_["seconds"],
+                          "ms_per_step": lr_["seconds"] * 1e3 / args.steps,
+                          "speedup_per_step": lv / value,
+                          "subvolume_simtime_per_s": simt_learned,
+                          "speedup_simulated_time": simt_learned / simt_plain if simt_plain else None,
+                          **{k: v for k, v in lr_.items() if k not in ("seconds",)}}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl, 2024, args.cpu_sample_steps)
     if args.extra:

@@ -28,7 +28,7 @@ def run_case(name, tmp_path, extra=""):
     src = os.path.join(CASES, name)
     for f in os.listdir(src):
         if f.endswith(".txt"):
-            shutil.copy(os.path.join(src, f), tmp_path / f)
+            shutil.copyfile(os.path.join(src, f), tmp_path / f)
     conf = (tmp_path / "conf.txt").read_text().rstrip("\n") + f"\nepsilon=0.05\nnc=10\nseed={SEED}\n{extra}"
     (tmp_path / "conf.txt").write_text(conf)
     print(f"driver case {name}", flush=True)
@@ -88,7 +88,7 @@ def test_log_freq_snapshots_do_not_change_the_trajectory(tmp_path):
     src = os.path.join(CASES, "react-9")
     for f in os.listdir(src):
         if f.endswith(".txt"):
-            shutil.copy(os.path.join(src, f), b / f)
+            shutil.copyfile(os.path.join(src, f), b / f)
     conf = (b / "conf.txt").read_text().replace("logFreq=0", "logFreq=1")
     (b / "conf.txt").write_text(conf.rstrip("\n") + f"\nepsilon=0.05\nnc=10\nseed={SEED}\n")
     r = subprocess.run([DRIVER, "conf.txt"], cwd=b, capture_output=True, text=True, timeout=110)

@@ -69,7 +69,7 @@ def test_driver_reports_bad_conf(tmp_path, mutation, msg):
     src = os.path.join(CASES, "react-9")
     for f in os.listdir(src):
         if f.endswith(".txt"):
-            shutil.copy(os.path.join(src, f), tmp_path / f)
+            shutil.copyfile(os.path.join(src, f), tmp_path / f)
     conf = (tmp_path / "conf.txt").read_text()
     (tmp_path / "conf.txt").write_text(mutation(conf))
     r = subprocess.run([DRIVER, "--parse-only", "conf.txt"], cwd=tmp_path, capture_output=True, text=True)
