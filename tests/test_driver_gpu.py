@@ -94,7 +94,7 @@ def test_log_freq_snapshots_do_not_change_the_trajectory(tmp_path):
     r = subprocess.run([DRIVER, "conf.txt"], cwd=b, capture_output=True, text=True, timeout=110)
     assert r.returncode == 0
     snaps = glob.glob(str(b / "sim*.dat"))
-    assert len(snaps) >= 8
+    assert len(snaps) >= 6  # one per >= 1 time unit after the previous one, plus the final
     final = max(snaps, key=lambda p: float(re.search(r"_([0-9.]+)\.dat$", p).group(1)))
     lines = open(final).read().split("\n")
     xb = np.array([[int(v) for v in lines[4 + i].split()] for i in range(9)], dtype=np.int64).T
