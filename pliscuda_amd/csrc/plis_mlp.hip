@@ -42,11 +42,16 @@ struct MlpDev {
     const float* norm;       // fmu[12] fsig[12] y_mu y_sig
 };
 
+// tanh(u) = 1 - 2 / (1 + e^{2u}) on the hardware exp / reciprocal (saturates
+// correctly for large |u|); the host reference uses the exact tanh.
+__device__ __forceinline__ float fast_tanh(float u) {
+    return 1.0f - 2.0f * __frcp_rn(1.0f + __expf(2.0f * u));
+}
 __device__ __forceinline__ float gelu_f(float z) {
-    return 0.5f * z * (1.0f + tanhf(kGeluC * (z + 0.044715f * z * z * z)));
+    return 0.5f * z * (1.0f + fast_tanh(kGeluC * (z + 0.044715f * z * z * z)));
 }
 __device__ __forceinline__ float gelu_d(float z) {
-    const float t = tanhf(kGeluC * (z + 0.044715f * z * z * z));
+    const float t = fast_tanh(kGeluC * (z + 0.044715f * z * z * z));
     return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * kGeluC * (1.0f + 3.0f * 0.044715f * z * z);
 }
 
@@ -84,11 +89,11 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// LDS of the inference-only path (one wavefront)
+// LDS of the inference-only path (one wavefront): one activation buffer is
+// reused by both hidden layers.
 struct MlpLdsInfer {
     __bf16 x[64 * kLdX];
-    __bf16 h1[64 * kLdH], h2[64 * kLdH];
-    float out[64 * 2];
+    __bf16 h[64 * kLdH];
 };
 
 // Standardise lane's feature row into LDS (lane r owns row r).
@@ -375,6 +380,46 @@ __global__ __launch_bounds__(256) void mlp_pack_kernel(const float* w32, __bf16*
     q[e] = (__bf16)v;
 }
 
+// Inference forward of the tile (one wavefront): layers 1-2 on MFMA with one
+// shared LDS buffer (layer 2 reads all its A fragments before overwriting),
+// the 64 -> 2 output layer on the VALU from the lane's own row.
+__device__ __forceinline__ void mlp_infer_tile(MlpLdsInfer& s, const MlpDev& m, float& o0, float& o1) {
+    wave_sync();
+    dense<16, 2, true>(s.x, kLdX, m.q + kQW1t, m.w32 + kPB1, s.h, nullptr, nullptr, nullptr);
+    wave_sync();
+    bf16x8 af[2][4];
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb) af[mb][kb] = lds_frag(s.h, kLdH, mb * 32, kb * 16);
+    wave_sync();
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb) {
+            f32x16 c = {};
+#pragma unroll
+            for (int kb = 0; kb < 4; ++kb) c = mfma(af[mb][kb], glb_frag(m.q + kQW2t, kNh, nb * 32, kb * 16), c);
+            const int col = nb * 32 + acc_col();
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const int row = mb * 32 + acc_row(i);
+                s.h[row * kLdH + col] = (__bf16)gelu_f(c[i] + m.w32[kPB2 + col]);
+            }
+        }
+    wave_sync();
+    const int r = threadIdx.x & 63;
+    float a0 = 0.0f, a1 = 0.0f;
+#pragma unroll 8
+    for (int k = 0; k < kNh; ++k) {
+        const float h = (float)s.h[r * kLdH + k];
+        a0 = fmaf(h, (float)m.q[kQW3 + k * kNoPad + 0], a0);
+        a1 = fmaf(h, (float)m.q[kQW3 + k * kNoPad + 1], a1);
+    }
+    o0 = a0 + m.w32[kPB3 + 0];
+    o1 = a1 + m.w32[kPB3 + 1];
+}
+
 // Inference over given feature rows (parity checks): out[n][2].
 __global__ __launch_bounds__(64) void mlp_infer_kernel(MlpDev m, const float* feats, uint32_t n, float* out) {
     __shared__ MlpLdsInfer s;
@@ -386,7 +431,7 @@ __global__ __launch_bounds__(64) void mlp_infer_kernel(MlpDev m, const float* fe
         for (int k = 0; k < 12; ++k) f[k] = r < n ? feats[(size_t)r * 12 + k] : 0.f;
         stage_inputs(s.x, nullptr, m, f, r < n);
         float o0, o1;
-        mlp_forward_tile(s.x, s.h1, nullptr, s.h2, nullptr, nullptr, nullptr, s.out, m, o0, o1);
+        mlp_infer_tile(s, m, o0, o1);
         if (r < n) {
             out[(size_t)r * 2] = o0;
             out[(size_t)r * 2 + 1] = o1;

@@ -75,7 +75,7 @@ def test_records_match_oracle():
     feats, ylog, crit = ctx.records(0)
     m = ffi.Model(net, topo)
     of = ffi.features(m, x0, 0.03, 10, 0.0)
-    assert np.array_equal(feats, of)
+    np.testing.assert_allclose(feats, of, rtol=2e-6, atol=2e-6)  # hardware log on the device
     tp, a0c, _ = ffi.tau_all(m, x0, 0.03, 10, 3)
     L = ffi.lib()
     ref = np.array([L.orc_log(float(t)) if t < 1e6 else L.orc_log(1e6) for t in tp], dtype=np.float32)
