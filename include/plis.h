@@ -141,6 +141,13 @@ int plis_debug_tau(plis_ctx *ctx, double *tau_prime, double *a0c, double *tau2);
 /* Launch shape in use: out[0] waves per block, out[1] blocks, out[2] tiles. */
 int plis_launch_shape(plis_ctx *ctx, int32_t *out);
 
+/* Diagnostic (no counterpart in the reference): number of tiles the
+ * edge-parallel leap passes of this context have handed to their redo pass
+ * since plis_create, i.e. tiles holding a Poisson draw of mean >= 10 or more
+ * undecided fast-path draws than the per-wavefront queue holds.  Results do
+ * not depend on it. */
+int plis_redo_tiles(plis_ctx *ctx, uint64_t *total);
+
 /* Autotune waves-per-block x blocks-per-CU on this context's workload by
  * timing `iters` real iterations per candidate (results do not depend on
  * the launch shape, so the trajectory is unchanged). */

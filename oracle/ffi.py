@@ -62,6 +62,16 @@ def _declare(L):
     L.orc_logfact.argtypes = [C.c_int64]
     L.orc_logfact.restype = C.c_double
     L.orc_poisson_many.argtypes = [C.c_double, C.c_uint64, C.c_uint32, P(C.c_int64)]
+    L.orc_poisson.argtypes = [C.c_double, C.c_uint32, P(C.c_uint32), P(C.c_uint32), C.c_uint32]
+    L.orc_poisson.restype = C.c_int64
+    L.orc_sq_exp_neg.argtypes = [C.c_float]
+    L.orc_sq_exp_neg.restype = C.c_float
+    L.orc_sq_decide_band.argtypes = [C.c_float, C.c_float, C.c_float]
+    L.orc_sq_decide_band.restype = C.c_int
+    L.orc_sq_poisson.argtypes = [C.c_double, C.c_uint32]
+    L.orc_sq_poisson.restype = C.c_int
+    L.orc_sq_maxerr.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
+    L.orc_sq_maxerr.restype = C.c_double
     L.orc_tau_all.argtypes = [P(Network), P(Topology), P(Params), P(C.c_uint32), C.c_uint32,
                               C.c_uint32, P(C.c_double), P(C.c_double), P(C.c_double)]
     L.orc_run.argtypes = [P(Network), P(Topology), P(Params), C.c_uint32, P(C.c_uint32),
@@ -149,6 +159,28 @@ def poisson_many(lam, seed, n):
     out = np.empty(n, dtype=np.int64)
     lib().orc_poisson_many(lam, seed, n, _ptr(out, C.c_int64))
     return out
+
+
+def poisson_inversion(lam, u0):
+    """The oracle's draw for mean lam < 10 from packed word u0 (the search
+    does not use the PTRS stream, so the counter is irrelevant)."""
+    z = (C.c_uint32 * 4)()
+    k = (C.c_uint32 * 2)()
+    return int(lib().orc_poisson(lam, u0, z, k, 0))
+
+
+def squeeze_poisson(lam, u0):
+    """The kernels' fast path for one inversion draw: k, or -1 if left to
+    the double search."""
+    return int(lib().orc_sq_poisson(lam, u0))
+
+
+def squeeze_decide(l, u, band):
+    return int(lib().orc_sq_decide_band(l, u, band))
+
+
+def squeeze_maxerr(b0, b1, stride=1):
+    return float(lib().orc_sq_maxerr(b0, b1, stride))
 
 
 def tau_all(model, x, eps, nc, seed, realization=0, step=0):

@@ -209,6 +209,97 @@ void orc_poisson_many(double lam, uint64_t seed, uint32_t n, int64_t *out)
     }
 }
 
+/* ---- check of the kernels' single-precision squeeze -----------------
+ * The kernels decide most inversion draws (mean < 10) in float: G_k, the
+ * CDF accumulated in single precision at the float-rounded mean, is
+ * compared with the float-rounded uniform with a guard band SQ_DELTA, and
+ * only a uniform inside the band (or k > SQ_KMAX) goes to the double
+ * search above.  The functions below restate that float evaluation with the
+ * same IEEE operations (fmaf, rintf, round-to-nearest products and sums) so
+ * that the tests can (1) bound max |G_k - F_k| over every float mean, which
+ * is what makes the squeeze return the double search's result, and
+ * (2) compare decisions draw by draw.  Checker only: orc_poisson above is
+ * the reference's draw. */
+#define SQ_DELTA 0x1p-15f
+#define SQ_KMAX 15
+static const float sq_recip[SQ_KMAX + 1] = {
+    0.0f,           0x1p+0f,        0x1p-1f,        0x1.555556p-2f, 0x1p-2f,        0x1.99999ap-3f,
+    0x1.555556p-3f, 0x1.24924ap-3f, 0x1p-3f,        0x1.c71c72p-4f, 0x1.99999ap-4f, 0x1.745d18p-4f,
+    0x1.555556p-4f, 0x1.3b13b2p-4f, 0x1.24924ap-4f, 0x1.111112p-4f};
+
+float orc_sq_exp_neg(float l)
+{
+    float n = rintf(l * 0x1.715476p+0f);
+    float r = fmaf(n, 0x1.62e4p-1f, -l);
+    r = fmaf(n, 0x1.7f7d1cp-20f, r);
+    float p = 0x1.a01a02p-13f;
+    p = fmaf(p, r, 0x1.6c16c2p-10f);
+    p = fmaf(p, r, 0x1.111112p-7f);
+    p = fmaf(p, r, 0x1.555556p-5f);
+    p = fmaf(p, r, 0x1.555556p-3f);
+    p = fmaf(p, r, 0.5f);
+    p = fmaf(p, r, 1.0f);
+    p = fmaf(p, r, 1.0f);
+    union { int32_t i; float f; } sc;
+    sc.i = (127 - (int32_t)n) << 23;
+    return p * sc.f;
+}
+
+/* band = SQ_DELTA is the kernels' squeeze; other bands let the tests show
+ * what a narrower guard band would get wrong. */
+int orc_sq_decide_band(float l, float u, float band)
+{
+    float p = orc_sq_exp_neg(l);
+    float G = p;
+    for (int k = 0;; ++k) {
+        if (u <= G - band) return k;
+        if (!(u > G + band) || k == SQ_KMAX) return -1;
+        p = (p * l) * sq_recip[k + 1];
+        G = G + p;
+    }
+}
+
+int orc_sq_decide(float l, float u) { return orc_sq_decide_band(l, u, SQ_DELTA); }
+
+/* The kernels' full inversion fast path for one draw: 0 from the exact
+ * zero shortcut, else the squeeze; -1 = left to the double search. */
+int orc_sq_poisson(double lam, uint32_t u0)
+{
+    if (!(lam > 0.0) || !(lam < POISSON_INV_MAX)) return -1;
+    double u = uni(u0);
+    if (u <= (1.0 - lam) - 0x1p-50) return 0;
+    return orc_sq_decide((float)lam, (float)u);
+}
+
+/* max over float means l with bit patterns b0, b0+stride, ... < b1 and over
+ * k <= SQ_KMAX of |G_k(l) - F_k(l)|, F_k the double search's CDF. */
+double orc_sq_maxerr(uint32_t b0, uint32_t b1, uint32_t stride)
+{
+    double worst = 0.0;
+    int64_t n = ((int64_t)b1 - (int64_t)b0 + stride - 1) / stride;
+#pragma omp parallel for schedule(static, 65536) reduction(max : worst)
+    for (int64_t i = 0; i < n; i++) {
+        union { uint32_t b; float f; } cv;
+        cv.b = b0 + (uint32_t)i * stride;
+        float l = cv.f;
+        double lam = (double)l;
+        float gp = orc_sq_exp_neg(l);
+        float G = gp;
+        double p = orc_exp(-lam);
+        double F = p;
+        for (int k = 0;; ++k) {
+            double e = fabs((double)G - F);
+            if (e > worst) worst = e;
+            if (k == SQ_KMAX) break;
+            gp = (gp * l) * sq_recip[k + 1];
+            G = G + gp;
+            p = (p * lam) / (double)(k + 1);
+            F = F + p;
+        }
+    }
+    return worst;
+}
+
 /* ------------------------------------------------------------------ */
 /* model preprocessing                                                 */
 /* ------------------------------------------------------------------ */

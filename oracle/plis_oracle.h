@@ -68,6 +68,15 @@ int64_t orc_poisson(double lam, uint32_t u0, const uint32_t ctr[4], const uint32
 /* Convenience: n draws with mean lam from distinct REACT streams (for
  * distribution tests).  out[i] uses subvolume id i. */
 void orc_poisson_many(double lam, uint64_t seed, uint32_t n, int64_t *out);
+/* Check of the kernels' float squeeze of the inversion search (see the
+ * comment in plis_oracle.c): float e^-l, the squeeze decision (k or -1),
+ * the kernels' whole fast path for one draw, and the worst |G_k - F_k| over
+ * float means with bits b0, b0+stride, ... < b1. */
+float orc_sq_exp_neg(float l);
+int orc_sq_decide(float l, float u);
+int orc_sq_decide_band(float l, float u, float band);
+int orc_sq_poisson(double lam, uint32_t u0);
+double orc_sq_maxerr(uint32_t b0, uint32_t b1, uint32_t stride);
 
 /* ---- per-subvolume kernels (for kernel-level parity) ---- */
 /* tau' and a0c for every subvolume of state x (SoA [s*N+v]); tau2 uses the

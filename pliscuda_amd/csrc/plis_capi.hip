@@ -30,7 +30,7 @@ struct plis_ctx {
     NetHeader nh{};
     int variant = 0;  // index into kVariantS / kVariantR
     int32_t S = 0, R = 0, C = 1;
-    uint32_t nreal = 0, ntiles = 0, ngroups = 0;
+    uint32_t nreal = 0, ntiles = 0;
     uint64_t total_counts = 0, total_sub = 0;
     std::vector<RealInfo> info;
     std::vector<RealCtl> ctl_host;
@@ -44,9 +44,10 @@ struct plis_ctx {
     uint32_t *d_rp = nullptr, *d_col = nullptr, *d_irp = nullptr, *d_icol = nullptr;
     float *d_w = nullptr, *d_iw = nullptr;
     int32_t* d_comp = nullptr;
-    PartA *d_pa_t = nullptr, *d_pa_g = nullptr;
-    PartB *d_pb_t = nullptr, *d_pb_g = nullptr;
-    uint32_t *d_cnt = nullptr;  // cntA_grp | cntA_real | cntB_grp | cntB_real | ndone
+    PartA* d_pa_t = nullptr;
+    PartB* d_pb_t = nullptr;
+    uint32_t* d_cnt = nullptr;   // [0] realizations done, [1] redo-list length, [2..3] redone tiles (u64)
+    uint32_t* d_redo = nullptr;  // redo list of the leap pass (tile ids)
     uint32_t* h_ndone = nullptr;  // pinned
     double eps = 0.03;
     int32_t nc = 10;
@@ -260,14 +261,11 @@ static KernelArgs make_args(plis_ctx* c) {
     a.iw = c->d_iw;
     a.comp = c->d_comp;
     a.partA_tile = c->d_pa_t;
-    a.partA_grp = c->d_pa_g;
     a.partB_tile = c->d_pb_t;
-    a.partB_grp = c->d_pb_g;
-    a.cntA_grp = c->d_cnt;
-    a.cntA_real = c->d_cnt + c->ngroups;
-    a.cntB_grp = c->d_cnt + c->ngroups + c->nreal;
-    a.cntB_real = c->d_cnt + 2 * c->ngroups + c->nreal;
-    a.ndone = c->d_cnt + 2 * c->ngroups + 2 * c->nreal;
+    a.ndone = c->d_cnt;
+    a.redo_count = c->d_cnt + 1;
+    a.redo_tiles = c->d_redo;
+    a.redo_total = reinterpret_cast<unsigned long long*>(c->d_cnt + 2);
     a.epsilon = c->eps;
     a.n_critical = c->nc;
     a.seed_lo = (uint32_t)c->seed;
@@ -305,6 +303,7 @@ static void launch_variant(plis_ctx* c, int which, const KernelArgs& a) {
             if constexpr (MS <= 8) {
                 if (c->edge_parallel) {
                     hipLaunchKernelGGL((phase_b_kernel<MS, MR, true>), grid, block, 0, c->stream, a);
+                    hipLaunchKernelGGL((phase_b_redo_kernel<MS, MR>), grid, block, 0, c->stream, a);
                     break;
                 }
             }
@@ -445,7 +444,7 @@ extern "C" int plis_create(const plis_network* net, int32_t n_real, const plis_t
     std::vector<TileEnt> tiles;
     bool any_weighted = false, any_asym = false;
     uint64_t xoff = 0, eoff = 0, rpoff = 0, ieoff = 0, irpoff = 0, coff = 0;
-    uint32_t tile = 0, group = 0;
+    uint32_t tile = 0;
     for (int32_t r = 0; r < n_real; ++r) {
         const plis_topology& t = topos[r];
         const uint32_t N = t.n_subvolumes;
@@ -498,11 +497,8 @@ extern "C" int plis_create(const plis_network* net, int32_t n_real, const plis_t
         hx.insert(hx.end(), x0[r], x0[r] + (size_t)S * N);
         ri.tile_begin = tile;
         ri.ntiles = (N + kTile - 1) / kTile;
-        ri.group_begin = group;
-        ri.ngroups = (ri.ntiles + kGroup - 1) / kGroup;
         for (uint32_t q = 0; q < ri.ntiles; ++q) tiles.push_back(TileEnt{(uint32_t)r, q * (uint32_t)kTile});
         tile += ri.ntiles;
-        group += ri.ngroups;
         xoff += (uint64_t)S * N;
         eoff += g.col.size();
         rpoff += N + 1;
@@ -510,7 +506,6 @@ extern "C" int plis_create(const plis_network* net, int32_t n_real, const plis_t
         c->total_sub += N;
     }
     c->ntiles = tile;
-    c->ngroups = group;
     c->total_counts = xoff;
 
     hipSetDevice(0);
@@ -532,7 +527,7 @@ extern "C" int plis_create(const plis_network* net, int32_t n_real, const plis_t
         rc.ev_idx = -1;
     }
     int rc = 0;
-    std::vector<uint32_t> zeros_cnt(2 * (size_t)c->ngroups + 2 * (size_t)c->nreal + 1, 0u);
+    std::vector<uint32_t> zeros_cnt(4, 0u);
     if ((rc = upload(c, &c->d_net_i, I)) || (rc = upload(c, &c->d_net_d, D)) ||
         (rc = upload(c, &c->d_info, c->info)) || (rc = upload(c, &c->d_ctl, c->ctl_host)) ||
         (rc = upload(c, &c->d_tiles, tiles)) || (rc = upload(c, &c->d_x0, hx)) ||
@@ -541,8 +536,8 @@ extern "C" int plis_create(const plis_network* net, int32_t n_real, const plis_t
         (rc = upload(c, &c->d_w, any_weighted ? hw : std::vector<float>())) ||
         (rc = upload(c, &c->d_irp, hirp)) || (rc = upload(c, &c->d_icol, hicol)) ||
         (rc = upload(c, &c->d_iw, hiw)) || (rc = upload(c, &c->d_comp, hcomp)) ||
-        (rc = dev_alloc(c, &c->d_pa_t, c->ntiles)) || (rc = dev_alloc(c, &c->d_pa_g, c->ngroups)) ||
-        (rc = dev_alloc(c, &c->d_pb_t, c->ntiles)) || (rc = dev_alloc(c, &c->d_pb_g, c->ngroups)) ||
+        (rc = dev_alloc(c, &c->d_pa_t, c->ntiles)) || (rc = dev_alloc(c, &c->d_pb_t, c->ntiles)) ||
+        (rc = dev_alloc(c, &c->d_redo, c->ntiles)) ||
         (rc = upload(c, &c->d_cnt, zeros_cnt))) {
         g_create_err = c->err;
         plis_destroy(c);
@@ -585,7 +580,7 @@ static int arm(plis_ctx* c, double t_end, double t_pause, uint64_t step_limit, u
     }
     HIPCHK(c, hipMemcpyAsync(c->d_ctl, c->ctl_host.data(), c->ctl_host.size() * sizeof(RealCtl),
                              hipMemcpyHostToDevice, c->stream));
-    uint32_t* d_ndone = c->d_cnt + 2 * c->ngroups + 2 * c->nreal;
+    uint32_t* d_ndone = c->d_cnt;
     HIPCHK(c, hipMemcpyAsync(d_ndone, &ndone, sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     *ndone_out = ndone;
@@ -602,7 +597,7 @@ extern "C" int plis_run_until(plis_ctx* c, double t_end, double t_pause, uint64_
     if (!c) return PLIS_ERR_ARG;
     uint32_t ndone = 0;
     if (int rc = arm(c, t_end, t_pause, step_limit, &ndone)) return rc;
-    uint32_t* d_ndone = c->d_cnt + 2 * c->ngroups + 2 * c->nreal;
+    uint32_t* d_ndone = c->d_cnt;
     uint32_t k = 2;
     uint64_t polls = 0, last_progress = 0;
     auto progress = [&]() {
@@ -762,6 +757,13 @@ extern "C" int plis_launch_shape(plis_ctx* c, int32_t* out) {
     out[0] = (int32_t)c->wpb;
     out[1] = (int32_t)c->blocks;
     out[2] = (int32_t)c->ntiles;
+    return PLIS_OK;
+}
+
+extern "C" int plis_redo_tiles(plis_ctx* c, uint64_t* total) {
+    if (!c || !total) return PLIS_ERR_ARG;
+    HIPCHK(c, hipMemcpyAsync(total, c->d_cnt + 2, sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return PLIS_OK;
 }
 

@@ -409,6 +409,7 @@ __global__ __launch_bounds__(256) void phase_a_kernel(KernelArgs a) {
 template <int MS, int MR>
 __global__ __launch_bounds__(256) void decide_kernel(KernelArgs a) {
     const uint32_t r = blockIdx.x;
+    if (r == 0 && threadIdx.x == 0) *a.redo_count = 0u;  // the coming leap pass's redo list
     if (r >= a.nreal) return;
     const RealInfo ri = a.info[r];
     if (a.ctl[r].done || a.ctl[r].retry) return;
@@ -641,39 +642,280 @@ __device__ __forceinline__ bool leap_subvolume(const KernelArgs& a, const NetVie
     return bad;
 }
 
-// Reactions, the critical event and the negativity check of one subvolume
-// whose hop balance (in-hops minus out-hops) is already known.
-template <int MS, int MR>
-__device__ __forceinline__ bool finish_subvolume(const KernelArgs& a, const NetView& n, const RealInfo& ri,
-                                                 uint32_t* Xn, uint32_t v, const uint32_t (&x)[MS], int c,
-                                                 const long long (&hop)[MS], const StepView& sv,
-                                                 unsigned long long& fire, uint32_t& viol) {
-    const uint32_t N = ri.N;
-    double av[MR];
-    uint32_t crit[(MR + 31) / 32];
-    react_props<MR, MS>(n, x, c, a.n_critical, av, crit);
-    long long nx[MS];
+// ---------------------------------------------------------------------------
+// Edge-parallel leap of one tile (symmetric realizations).
+//
+// The tile's CSR edges are contiguous; lane l takes edges l, l+64, ...
+// (coalesced neighbour-id and weight loads, no divergence from uneven
+// degrees), finds the owning subvolume by binary search over the tile's row
+// pointers staged in LDS, and draws both hop directions of every species
+// from the edge's Philox blocks; then lane i draws the reactions of
+// subvolume v0+i.  Every balance goes into LDS with integer atomics (sums
+// are order-free, so this is exact).
+//
+// Two tiers of draws.  phase_b_kernel's walk runs only poisson_fast (zero
+// shortcut and float squeeze); a draw it cannot settle is appended to the
+// wavefront's LDS queue as a 32-bit code and drawn after the walk with the
+// double search, inline, where almost nothing else is live.  A tile whose
+// queue overflows, or that holds a draw of mean >= 10 (PTRS), writes
+// nothing: it goes on the redo list and phase_b_redo_kernel draws it again
+// from scratch with the complete sampler (FULL).  Draws are functions of
+// their counters, so the redo gives the tile exactly the values a single
+// pass would.  No call and no PTRS in phase_b_kernel keeps its register
+// footprint, and so its occupancy, at that of the fast path.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kQueue = 128;               // codes per wavefront
+constexpr uint32_t kQueueReact = 0x80000000u;  // reaction code: flag | lane << 8 | j
+constexpr uint32_t kQueueDefer = kQueue + 1u;  // count bump of a draw the queue does not take
+
+__device__ __forceinline__ void lds_wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ void q_append(uint32_t* s_q, uint32_t* s_qn, double lam, uint32_t code) {
+    if (!(lam < kPoissonInvMax)) {
+        atomicAdd(s_qn, kQueueDefer);
+        return;
+    }
+    const uint32_t slot = atomicAdd(s_qn, 1u);
+    if (slot < kQueue) s_q[slot] = code;
+}
+
+// Owner of edge e of the tile: last i with s_rp[i] <= e.
+__device__ __forceinline__ uint32_t edge_owner(const uint32_t* s_rp, uint32_t e) {
+    uint32_t lo_i = 0, hi_i = kTile - 1;
 #pragma unroll
-    for (int s = 0; s < MS; ++s) nx[s] = (long long)x[s] + hop[s];
+    for (int it = 0; it < 6; ++it) {
+        const uint32_t mid = (lo_i + hi_i + 1u) >> 1;
+        if (s_rp[mid] <= e) lo_i = mid;
+        else hi_i = mid - 1u;
+    }
+    return lo_i;
+}
+
+__device__ __forceinline__ double hop_mean(double d, uint32_t x, double wq, double tau, int nc) {
+    return hop_active(d, x, nc) ? tau * ((d * (double)x) * wq) : 0.0;
+}
+
+// One draw of the walk: complete (FULL) or fast with the queue; returns the
+// count to apply now (0 for a queued draw).
+template <bool FULL>
+__device__ __forceinline__ long long walk_draw(double lam, const U4& o, const DrawId& id, uint32_t* s_q,
+                                               uint32_t* s_qn, uint32_t code) {
+    if constexpr (FULL) {
+        return poisson(lam, pick(o, id.word), id);
+    } else {
+        const int k = poisson_fast(lam, pick(o, id.word));
+        if (k >= 0) return k;
+        q_append(s_q, s_qn, lam, code);
+        return 0;
+    }
+}
+
+// The lane's draws: its edges, then the reactions of its subvolume.
+template <int MS, int MR, bool FULL>
+__device__ __forceinline__ void walk_tile_draws(const KernelArgs& a, const NetView& n, const RealInfo& ri,
+                                                const uint32_t* X, uint32_t v0, const StepView& sv,
+                                                uint32_t (*s_x)[kTile], unsigned long long (*s_del)[kTile],
+                                                const uint32_t* s_rp, const int32_t* s_c, uint32_t* s_q,
+                                                uint32_t* s_qn, unsigned long long& fire) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t N = ri.N;
+    const int nc = a.n_critical;
+    const bool weighted = (ri.flags & 2u) != 0;
+    const uint32_t* colp = a.col + ri.e_off;
+    const float* wp = a.w + ri.e_off;
     const double tau = sv.tau;
     const uint32_t k0 = a.seed_lo, k1 = a.seed_hi;
+    const uint32_t c3hop = ctr_word3(sv.real, sv.attempt, kHop, 0u);
+    const uint32_t eb = s_rp[0], ee = s_rp[kTile];
+    for (uint32_t e = eb + lane; e < ee; e += (uint32_t)kTile) {
+        const uint32_t i = edge_owner(s_rp, e);
+        const uint32_t vo = v0 + i;
+        const uint32_t u = colp[e];
+        const double wq = weighted ? (double)wp[e] : 1.0;
+        const int co = s_c[i];
+        const int cu = n.C > 1 ? a.comp[ri.comp_off + u] : 0;
+        const uint32_t lo = vo < u ? vo : u, hi = vo < u ? u : vo;
+        const uint32_t dout = vo > u ? 1u : 0u, din = 1u - dout;
+        const uint32_t ecode = (e - eb) << 4;
+#pragma unroll
+        for (int s0 = 0; s0 < MS; s0 += 2) {
+            if (s0 >= n.S) break;
+            const bool two = s0 + 1 < MS && s0 + 1 < n.S;
+            const int s1 = s0 + 1 < MS ? s0 + 1 : s0;
+            const uint32_t xo0 = s_x[s0][i], xo1 = two ? s_x[s1][i] : 0u;
+            const double do0 = n.d[co * n.S + s0], do1 = two ? n.d[co * n.S + s1] : 0.0;
+            const uint32_t xu0 = X[ri.x_off + (uint64_t)s0 * N + u];
+            const uint32_t xu1 = two ? X[ri.x_off + (uint64_t)s1 * N + u] : 0u;
+            const double du0 = n.d[cu * n.S + s0], du1 = two ? n.d[cu * n.S + s1] : 0.0;
+            const double lo0 = hop_mean(do0, xo0, wq, tau, nc);
+            const double lo1 = two ? hop_mean(do1, xo1, wq, tau, nc) : 0.0;
+            const double li0 = hop_mean(du0, xu0, wq, tau, nc);
+            const double li1 = two ? hop_mean(du1, xu1, wq, tau, nc) : 0.0;
+            if (!(lo0 > 0.0 || lo1 > 0.0 || li0 > 0.0 || li1 > 0.0)) continue;
+            const uint32_t blk = (uint32_t)s0 >> 1;
+            const uint32_t cb = c3hop | blk;
+            const U4 o = philox10(lo, hi, sv.step, cb, k0, k1);
+            const uint32_t wcode = ecode | (blk << 2);
+            long long b0 = 0, b1 = 0;
+            if (lo0 > 0.0) {
+                const long long k = walk_draw<FULL>(lo0, o, DrawId{lo, hi, sv.step, cb, k0, k1, dout}, s_q, s_qn,
+                                                    wcode | dout);
+                b0 -= k;
+                fire += (unsigned long long)k;
+            }
+            if (li0 > 0.0)
+                b0 += walk_draw<FULL>(li0, o, DrawId{lo, hi, sv.step, cb, k0, k1, din}, s_q, s_qn, wcode | din);
+            if (lo1 > 0.0) {
+                const long long k = walk_draw<FULL>(lo1, o, DrawId{lo, hi, sv.step, cb, k0, k1, 2u + dout}, s_q,
+                                                    s_qn, wcode | (2u + dout));
+                b1 -= k;
+                fire += (unsigned long long)k;
+            }
+            if (li1 > 0.0)
+                b1 += walk_draw<FULL>(li1, o, DrawId{lo, hi, sv.step, cb, k0, k1, 2u + din}, s_q, s_qn,
+                                      wcode | (2u + din));
+            if (b0 != 0) atomicAdd(&s_del[s0][i], (unsigned long long)b0);
+            if (b1 != 0) atomicAdd(&s_del[s1][i], (unsigned long long)b1);
+        }
+    }
+    const uint32_t v = v0 + lane;
+    if (v >= N) return;
+    // reactions of subvolume v, propensities and criticality as react_props
+    // computes them (same operations, one reaction at a time)
+    const int c = s_c[lane];
     uint32_t blk = 0xFFFFFFFFu;
     U4 o{0, 0, 0, 0};
     const uint32_t c3base = ctr_word3(sv.real, sv.attempt, kReact, 0u);
     for (int j = 0; j < n.R; ++j) {
-        if (is_crit<MR>(crit, j) || !(av[j] > 0.0)) continue;
+        double aj = n.k[c * n.R + j];
+        long long L = -1;
+        const int nr = n.nrs[j];
+        for (int q = 0; q < nr; ++q) {
+            const int sq = n.rs[j * 3 + q], rq = n.rr[j * 3 + q];
+            const uint32_t xs = s_x[sq][lane];
+            aj = aj * binom_d(xs, rq);
+            const long long l = (long long)(xs / (uint32_t)rq);
+            if (L < 0 || l < L) L = l;
+        }
+        if ((n.order[j] > 0 && L < (long long)nc) || !(aj > 0.0)) continue;
         const uint32_t b = (uint32_t)j >> 2;
         if (b != blk) {
             o = philox10(v, kSelfId, sv.step, c3base | b, k0, k1);
             blk = b;
         }
         const uint32_t wd = (uint32_t)j & 3u;
-        const long long k = poisson(tau * av[j], pick(o, wd), DrawId{v, kSelfId, sv.step, c3base | b, k0, k1, wd});
+        const long long k = walk_draw<FULL>(tau * aj, o, DrawId{v, kSelfId, sv.step, c3base | b, k0, k1, wd}, s_q,
+                                            s_qn, kQueueReact | (lane << 8) | (uint32_t)j);
+        if (k == 0) continue;
         fire += (unsigned long long)k;
-#pragma unroll
-        for (int s = 0; s < MS; ++s)
-            if (s < n.S) nx[s] += k * (long long)n.nu[s * n.R + j];
+        for (int s = 0; s < n.S; ++s) {
+            const int nu = n.nu[s * n.R + j];
+            if (nu != 0) atomicAdd(&s_del[s][lane], (unsigned long long)(k * (long long)nu));
+        }
     }
+}
+
+// One queued draw (mean < 10), re-derived from its code; the double search.
+template <int MS>
+__device__ __forceinline__ void drain_draw(const KernelArgs& a, const NetView& n, const RealInfo& ri,
+                                           const uint32_t* X, uint32_t v0, const StepView& sv,
+                                           uint32_t (*s_x)[kTile], unsigned long long (*s_del)[kTile],
+                                           const uint32_t* s_rp, const int32_t* s_c, uint32_t code,
+                                           unsigned long long& fire) {
+    const uint32_t k0 = a.seed_lo, k1 = a.seed_hi;
+    if (code & kQueueReact) {
+        const uint32_t i = (code >> 8) & 63u, j = code & 255u;
+        double aj = n.k[s_c[i] * n.R + (int)j];
+        const int nr = n.nrs[j];
+        for (int q = 0; q < nr; ++q) aj = aj * binom_d(s_x[n.rs[j * 3 + q]][i], n.rr[j * 3 + q]);
+        const U4 o = philox10(v0 + i, kSelfId, sv.step, ctr_word3(sv.real, sv.attempt, kReact, 0u) | (j >> 2), k0, k1);
+        const long long k = inversion_search(sv.tau * aj, pick(o, j & 3u));
+        if (k == 0) return;
+        fire += (unsigned long long)k;
+        for (int s = 0; s < n.S; ++s) {
+            const int nu = n.nu[s * n.R + (int)j];
+            if (nu != 0) atomicAdd(&s_del[s][i], (unsigned long long)(k * (long long)nu));
+        }
+        return;
+    }
+    const uint32_t e = s_rp[0] + (code >> 4), w = code & 15u;
+    const uint32_t blk = w >> 2, wd = w & 3u;
+    const uint32_t i = edge_owner(s_rp, e);
+    const uint32_t vo = v0 + i;
+    const uint32_t u = a.col[ri.e_off + e];
+    const double wq = (ri.flags & 2u) ? (double)a.w[ri.e_off + e] : 1.0;
+    const uint32_t lo = vo < u ? vo : u, hi = vo < u ? u : vo;
+    const uint32_t dout = vo > u ? 1u : 0u;
+    const int s = 2 * (int)blk + (int)(wd >> 1);
+    const bool out = (wd & 1u) == dout;
+    double lam;
+    if (out) {
+        lam = hop_mean(n.d[s_c[i] * n.S + s], s_x[s][i], wq, sv.tau, a.n_critical);
+    } else {
+        const int cu = n.C > 1 ? a.comp[ri.comp_off + u] : 0;
+        lam = hop_mean(n.d[cu * n.S + s], X[ri.x_off + (uint64_t)s * ri.N + u], wq, sv.tau, a.n_critical);
+    }
+    const U4 o = philox10(lo, hi, sv.step, ctr_word3(sv.real, sv.attempt, kHop, 0u) | blk, k0, k1);
+    const long long k = inversion_search(lam, pick(o, wd));
+    if (k == 0) return;
+    if (out) {
+        fire += (unsigned long long)k;
+        atomicAdd(&s_del[s][i], (unsigned long long)(-k));
+    } else {
+        atomicAdd(&s_del[s][i], (unsigned long long)k);
+    }
+}
+
+// The leap of one tile; false (nothing written) when the tile is left to
+// phase_b_redo_kernel.
+template <int MS, int MR, bool FULL>
+__device__ __forceinline__ bool leap_tile_edges(const KernelArgs& a, const NetView& n, const RealInfo& ri,
+                                                const uint32_t* X, uint32_t* Xn, uint32_t v0,
+                                                const StepView& sv, uint32_t (*s_x)[kTile],
+                                                unsigned long long (*s_del)[kTile], uint32_t* s_rp,
+                                                int32_t* s_c, uint32_t* s_q, uint32_t* s_qn,
+                                                unsigned long long& fire, uint32_t& bad, uint32_t& viol) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t N = ri.N;
+    const uint32_t v = v0 + lane;
+    const bool active = v < N;
+#pragma unroll
+    for (int s = 0; s < MS; ++s) {
+        s_x[s][lane] = (active && s < n.S) ? X[ri.x_off + (uint64_t)s * N + v] : 0u;
+        s_del[s][lane] = 0ull;
+    }
+    s_c[lane] = (n.C > 1 && active) ? a.comp[ri.comp_off + v] : 0;
+    s_rp[lane] = a.rp[ri.rp_off + min(v, N)];
+    if (lane == 0) {
+        s_rp[kTile] = a.rp[ri.rp_off + min(v0 + (uint32_t)kTile, N)];
+        *s_qn = 0u;
+    }
+    lds_wave_sync();
+    // hop codes hold the edge offset in 27 bits
+    if (!FULL && wave_uniform(s_rp[kTile] - s_rp[0]) >= (1u << 27)) return false;
+    walk_tile_draws<MS, MR, FULL>(a, n, ri, X, v0, sv, s_x, s_del, s_rp, s_c, s_q, s_qn, fire);
+    lds_wave_sync();
+    if constexpr (!FULL) {
+        const uint32_t cnt = wave_uniform(*s_qn);
+        if (cnt > kQueue) return false;
+        for (uint32_t q = lane; q < cnt; q += (uint32_t)kTile)
+            drain_draw<MS>(a, n, ri, X, v0, sv, s_x, s_del, s_rp, s_c, s_q[q], fire);
+        lds_wave_sync();
+    }
+    if (!active) return true;
+
+    uint32_t x[MS];
+    long long nx[MS];
+#pragma unroll
+    for (int s = 0; s < MS; ++s) {
+        x[s] = s_x[s][lane];
+        nx[s] = (long long)x[s] + (long long)s_del[s][lane];
+    }
+    // the critical event (one per global step, in the argmin-tau'' subvolume)
     if (sv.crit) {
         if (v == sv.vstar) {
             if (sv.ev_kind == 1) {
@@ -694,118 +936,16 @@ __device__ __forceinline__ bool finish_subvolume(const KernelArgs& a, const NetV
                 if (s == sv.ev_idx) nx[s] += 1;
         }
     }
-    bool bad = false;
+    bool neg = false;
 #pragma unroll
     for (int s = 0; s < MS; ++s) {
         if (s >= n.S) break;
-        bad = bad || nx[s] < 0 || nx[s] > 0xFFFFFFFFll;
+        neg = neg || nx[s] < 0 || nx[s] > 0xFFFFFFFFll;
         Xn[ri.x_off + (uint64_t)s * N + v] = (uint32_t)nx[s];
     }
+    bad = neg ? 1u : 0u;
     viol = leap_violation<MS>(n.hor, n.mult, n.S, x, nx, a.epsilon, a.theta);
-    return bad;
-}
-
-// Edge-parallel leap of one tile of a symmetric realization.  The tile's
-// CSR edges are contiguous; lane l takes edges l, l+64, ... (coalesced
-// neighbour-id and weight loads, no divergence from uneven degrees), finds
-// the owning subvolume by binary search over the tile's row pointers staged
-// in LDS, draws both hop directions of every species from the edge's Philox
-// blocks and adds the owner's integer balance into LDS (integer sums are
-// order-free, so this is exact).  Then lane i finishes subvolume v0+i.
-template <int MS, int MR>
-__device__ __forceinline__ void leap_tile_edges(const KernelArgs& a, const NetView& n, const RealInfo& ri,
-                                                const uint32_t* X, uint32_t* Xn, uint32_t v0,
-                                                const StepView& sv, uint32_t (*s_x)[kTile],
-                                                unsigned long long (*s_del)[kTile], uint32_t* s_rp,
-                                                int32_t* s_c, unsigned long long& fire, uint32_t& bad,
-                                                uint32_t& viol) {
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t N = ri.N;
-    const int nc = a.n_critical;
-    const uint32_t v = v0 + lane;
-    const bool active = v < N;
-    uint32_t x[MS];
-#pragma unroll
-    for (int s = 0; s < MS; ++s) {
-        x[s] = (active && s < n.S) ? X[ri.x_off + (uint64_t)s * N + v] : 0u;
-        s_x[s][lane] = x[s];
-        s_del[s][lane] = 0ull;
-    }
-    const int c = (n.C > 1 && active) ? a.comp[ri.comp_off + v] : 0;
-    s_c[lane] = c;
-    s_rp[lane] = a.rp[ri.rp_off + min(v, N)];
-    if (lane == 0) s_rp[kTile] = a.rp[ri.rp_off + min(v0 + (uint32_t)kTile, N)];
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-
-    const bool weighted = (ri.flags & 2u) != 0;
-    const uint32_t* colp = a.col + ri.e_off;
-    const float* wp = a.w + ri.e_off;
-    const double tau = sv.tau;
-    const uint32_t k0 = a.seed_lo, k1 = a.seed_hi;
-    const uint32_t c3hop = ctr_word3(sv.real, sv.attempt, kHop, 0u);
-    const uint32_t eb = s_rp[0], ee = s_rp[kTile];
-    for (uint32_t e = eb + lane; e < ee; e += (uint32_t)kTile) {
-        // owner: last i with s_rp[i] <= e
-        uint32_t lo_i = 0, hi_i = kTile - 1;
-#pragma unroll
-        for (int it = 0; it < 6; ++it) {
-            const uint32_t mid = (lo_i + hi_i + 1u) >> 1;
-            if (s_rp[mid] <= e) lo_i = mid;
-            else hi_i = mid - 1u;
-        }
-        const uint32_t i = lo_i;
-        const uint32_t vo = v0 + i;
-        const uint32_t u = colp[e];
-        const double wq = weighted ? (double)wp[e] : 1.0;
-        const int co = s_c[i];
-        const int cu = n.C > 1 ? a.comp[ri.comp_off + u] : 0;
-        const uint32_t lo = vo < u ? vo : u, hi = vo < u ? u : vo;
-        const uint32_t dout = vo > u ? 1u : 0u, din = 1u - dout;
-#pragma unroll
-        for (int s0 = 0; s0 < MS; s0 += 2) {
-            if (s0 >= n.S) break;
-            const bool two = s0 + 1 < MS && s0 + 1 < n.S;
-            const int s1 = s0 + 1 < MS ? s0 + 1 : s0;
-            const uint32_t xo0 = s_x[s0][i], xo1 = two ? s_x[s1][i] : 0u;
-            const double do0 = n.d[co * n.S + s0], do1 = two ? n.d[co * n.S + s1] : 0.0;
-            const uint32_t xu0 = X[ri.x_off + (uint64_t)s0 * N + u];
-            const uint32_t xu1 = two ? X[ri.x_off + (uint64_t)s1 * N + u] : 0u;
-            const double du0 = n.d[cu * n.S + s0], du1 = two ? n.d[cu * n.S + s1] : 0.0;
-            const double lo0 = hop_active(do0, xo0, nc) ? tau * ((do0 * (double)xo0) * wq) : 0.0;
-            const double lo1 = (two && hop_active(do1, xo1, nc)) ? tau * ((do1 * (double)xo1) * wq) : 0.0;
-            const double li0 = hop_active(du0, xu0, nc) ? tau * ((du0 * (double)xu0) * wq) : 0.0;
-            const double li1 = (two && hop_active(du1, xu1, nc)) ? tau * ((du1 * (double)xu1) * wq) : 0.0;
-            if (!(lo0 > 0.0 || lo1 > 0.0 || li0 > 0.0 || li1 > 0.0)) continue;
-            const uint32_t cb = c3hop | ((uint32_t)s0 >> 1);
-            const U4 o = philox10(lo, hi, sv.step, cb, k0, k1);
-            long long b0 = 0, b1 = 0;
-            if (lo0 > 0.0) {
-                const long long k = poisson(lo0, pick(o, dout), DrawId{lo, hi, sv.step, cb, k0, k1, dout});
-                b0 -= k;
-                fire += (unsigned long long)k;
-            }
-            if (li0 > 0.0) b0 += poisson(li0, pick(o, din), DrawId{lo, hi, sv.step, cb, k0, k1, din});
-            if (lo1 > 0.0) {
-                const long long k = poisson(lo1, pick(o, 2u + dout), DrawId{lo, hi, sv.step, cb, k0, k1, 2u + dout});
-                b1 -= k;
-                fire += (unsigned long long)k;
-            }
-            if (li1 > 0.0) b1 += poisson(li1, pick(o, 2u + din), DrawId{lo, hi, sv.step, cb, k0, k1, 2u + din});
-            if (b0 != 0) atomicAdd(&s_del[s0][i], (unsigned long long)b0);
-            if (b1 != 0) atomicAdd(&s_del[s1][i], (unsigned long long)b1);
-        }
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    if (active) {
-        long long hop[MS];
-#pragma unroll
-        for (int s = 0; s < MS; ++s) hop[s] = (long long)s_del[s][lane];
-        bad = finish_subvolume<MS, MR>(a, n, ri, Xn, v, x, c, hop, sv, fire, viol) ? 1u : 0u;
-    }
+    return true;
 }
 
 __device__ __forceinline__ void reduce_partB(unsigned long long& fire, uint32_t& bad, uint32_t& viol) {
@@ -870,53 +1010,87 @@ __device__ void finalize_B(const KernelArgs& a, const RealInfo& ri, uint32_t r, 
     a.ctl[r] = c;
 }
 
-// EP: edge-parallel tiles (every realization of the context is symmetric);
-// otherwise one lane walks one subvolume's out- and in-edges.
+// Phase B of one tile.  EP: edge-parallel tile (every realization of the
+// context is symmetric; FULL = complete sampler, the redo pass); otherwise
+// one lane walks one subvolume's out- and in-edges.
+template <int MS, int MR, bool EP, bool FULL>
+__device__ __forceinline__ void phase_b_tile(const KernelArgs& a, const NetView& n, uint32_t t,
+                                             uint32_t (*s_x)[kTile], unsigned long long (*s_del)[kTile],
+                                             uint32_t* s_rp, int32_t* s_c, uint32_t* s_q, uint32_t* s_qn) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const TileEnt te = a.tiles[t];
+    const uint32_t r = wave_uniform(te.r);
+    const RealInfo ri = a.info[r];
+    const RealCtl* cp = &a.ctl[r];
+    if (wave_uniform(cp->done)) return;
+    StepView sv;
+    sv.tau = cp->tau;
+    sv.step = wave_uniform(cp->step);
+    sv.attempt = wave_uniform(cp->attempt);
+    sv.real = ri.id;
+    sv.crit = (int)wave_uniform((uint32_t)cp->crit);
+    sv.ev_kind = (int)wave_uniform((uint32_t)cp->ev_kind);
+    sv.ev_idx = (int)wave_uniform((uint32_t)cp->ev_idx);
+    sv.vstar = wave_uniform(cp->vstar);
+    sv.ev_dst = wave_uniform(cp->ev_dst);
+    const uint32_t cur = wave_uniform(cp->cur);
+    const uint32_t* X = cur ? a.x1 : a.x0;
+    uint32_t* Xn = cur ? a.x0 : a.x1;
+    unsigned long long fire = 0;
+    uint32_t bad = 0, viol = 0;
+    if constexpr (EP) {
+        if (!leap_tile_edges<MS, MR, FULL>(a, n, ri, X, Xn, te.v0, sv, s_x, s_del, s_rp, s_c, s_q, s_qn, fire, bad,
+                                           viol)) {
+            if (lane == 0) {
+                const uint32_t slot = atomicAdd(a.redo_count, 1u);
+                if (slot < a.ntiles) a.redo_tiles[slot] = t;  // one entry per tile per pass
+            }
+            return;
+        }
+    } else {
+        const uint32_t v = te.v0 + lane;
+        if (v < ri.N) bad = leap_subvolume<MS, MR>(a, n, ri, X, Xn, v, sv, fire, viol) ? 1u : 0u;
+    }
+    reduce_partB(fire, bad, viol);
+    if (lane == 0) a.partB_tile[t] = PartB{fire, bad, viol};
+}
+
 template <int MS, int MR, bool EP>
 __global__ __launch_bounds__(256) void phase_b_kernel(KernelArgs a) {
     const NetView n = net_view(a);
-    const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wv = wave_uniform(threadIdx.x >> 6);
     // LDS staging for the edge-parallel path (up to 4 waves per block)
-    constexpr bool kEdgePar = EP;
-    constexpr int LS = kEdgePar ? MS : 1;
+    constexpr int LS = EP ? MS : 1;
     __shared__ uint32_t s_x[4][LS][kTile];
     __shared__ unsigned long long s_del[4][LS][kTile];
     __shared__ uint32_t s_rp[4][kTile + 1];
     __shared__ int32_t s_c[4][kTile];
+    __shared__ uint32_t s_q[4][EP ? kQueue : 1];
+    __shared__ uint32_t s_qn[4];
     for (uint32_t it = 0;; ++it) {
         const uint32_t t = tile_at(it, a.ntiles, a.waves_per_block);
         if (t == 0xFFFFFFFFu) break;
-        const TileEnt te = a.tiles[t];
-        const uint32_t r = wave_uniform(te.r);
-        const RealInfo ri = a.info[r];
-        const RealCtl* cp = &a.ctl[r];
-        if (wave_uniform(cp->done)) continue;
-        StepView sv;
-        sv.tau = cp->tau;
-        sv.step = wave_uniform(cp->step);
-        sv.attempt = wave_uniform(cp->attempt);
-        sv.real = ri.id;
-        sv.crit = (int)wave_uniform((uint32_t)cp->crit);
-        sv.ev_kind = (int)wave_uniform((uint32_t)cp->ev_kind);
-        sv.ev_idx = (int)wave_uniform((uint32_t)cp->ev_idx);
-        sv.vstar = wave_uniform(cp->vstar);
-        sv.ev_dst = wave_uniform(cp->ev_dst);
-        const uint32_t cur = wave_uniform(cp->cur);
-        const uint32_t* X = cur ? a.x1 : a.x0;
-        uint32_t* Xn = cur ? a.x0 : a.x1;
-        const uint32_t v = te.v0 + lane;
-        unsigned long long fire = 0;
-        uint32_t bad = 0, viol = 0;
-        if constexpr (kEdgePar) {
-            leap_tile_edges<MS, MR>(a, n, ri, X, Xn, te.v0, sv, s_x[wv], s_del[wv], s_rp[wv], s_c[wv], fire, bad,
-                                    viol);
-        } else {
-            if (v < ri.N) bad = leap_subvolume<MS, MR>(a, n, ri, X, Xn, v, sv, fire, viol) ? 1u : 0u;
-        }
-        reduce_partB(fire, bad, viol);
-        if (lane == 0) a.partB_tile[t] = PartB{fire, bad, viol};
+        phase_b_tile<MS, MR, EP, false>(a, n, t, s_x[wv], s_del[wv], s_rp[wv], s_c[wv], s_q[wv], &s_qn[wv]);
     }
+}
+
+// Tiles phase_b_kernel left on the redo list, drawn with the complete
+// sampler.  The list is emptied by decide_kernel before each leap pass.
+template <int MS, int MR>
+__global__ __launch_bounds__(256) void phase_b_redo_kernel(KernelArgs a) {
+    const NetView n = net_view(a);
+    const uint32_t wv = wave_uniform(threadIdx.x >> 6);
+    const uint32_t wpb = blockDim.x >> 6;
+    __shared__ uint32_t s_x[4][MS][kTile];
+    __shared__ unsigned long long s_del[4][MS][kTile];
+    __shared__ uint32_t s_rp[4][kTile + 1];
+    __shared__ int32_t s_c[4][kTile];
+    __shared__ uint32_t s_qn[4];
+    const uint32_t cnt = min(wave_uniform(*a.redo_count), a.ntiles);
+    if (blockIdx.x == 0 && threadIdx.x == 0 && cnt != 0u) *a.redo_total += cnt;
+    for (uint32_t q = blockIdx.x * wpb + wv; q < cnt; q += gridDim.x * wpb)
+        phase_b_tile<MS, MR, true, true>(a, n, wave_uniform(a.redo_tiles[q]), s_x[wv], s_del[wv], s_rp[wv], s_c[wv],
+                                         nullptr, &s_qn[wv]);
 }
 
 // One block per realization: sum of firings / any negative over the tile
@@ -969,7 +1143,9 @@ __global__ __launch_bounds__(256) void commit_kernel(KernelArgs a) {
     template __global__ void phase_a_kernel<MS, MR>(KernelArgs);            \
     template __global__ void decide_kernel<MS, MR>(KernelArgs);             \
     template __global__ void phase_b_kernel<MS, MR, false>(KernelArgs);
-#define PLIS_INST_EP(MS, MR) template __global__ void phase_b_kernel<MS, MR, true>(KernelArgs);
+#define PLIS_INST_EP(MS, MR)                                                \
+    template __global__ void phase_b_kernel<MS, MR, true>(KernelArgs);      \
+    template __global__ void phase_b_redo_kernel<MS, MR>(KernelArgs);
 PLIS_INST(1, 4)
 PLIS_INST(3, 4)
 PLIS_INST(5, 8)

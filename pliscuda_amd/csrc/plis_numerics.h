@@ -192,31 +192,101 @@ __device__ __attribute__((noinline)) long long poisson_ptrs(double lam, DrawId i
 // Slow path of the sampler: full sequential-search inversion (mean < 10) or
 // PTRS (mean >= 10).  Kept out of line so the common no-event case below
 // stays a handful of instructions at every call site.
-__device__ __attribute__((noinline)) long long poisson_slow(double lam, uint32_t u0, DrawId id) {
-    if (lam < kPoissonInvMax) {
-        const double u = uniform01(u0);
-        double p = dexp(-lam);
-        double F = p;
-        long long k = 0;
+__device__ __forceinline__ long long inversion_search(double lam, uint32_t u0) {
+    const double u = uniform01(u0);
+    double p = dexp(-lam);
+    double F = p;
+    long long k = 0;
 #pragma unroll 1
-        while (u > F && k < 1000) {
-            k += 1;
-            p = (p * lam) / (double)k;
-            F = F + p;
-        }
-        return k;
+    while (u > F && k < 1000) {
+        k += 1;
+        p = (p * lam) / (double)k;
+        F = F + p;
     }
+    return k;
+}
+
+__device__ __attribute__((noinline)) long long poisson_slow(double lam, uint32_t u0, DrawId id) {
+    if (lam < kPoissonInvMax) return inversion_search(lam, u0);
     return poisson_ptrs(lam, id);
+}
+
+// ---- single-precision squeeze of the inversion search ----
+// F_k is the double CDF that poisson_slow accumulates, G_k the float one
+// below, evaluated at the float-rounded mean.  For every float mean in
+// (0, 10) and k <= kSqueezeK, |G_k - F_k| stays below 2^-20 (checked over
+// all such floats on the host with the same operations); rounding the mean
+// moves F_k by at most 2^-21 (|dF_k/dlam| <= 1), rounding u by at most
+// 2^-25, and forming G -+ delta by 2^-25 more.  With delta = 2^-15 >> the
+// sum, u_f <= G_k - delta proves u <= F_k and u_f > G_k + delta proves
+// u > F_k: the search result is found without double arithmetic, and is
+// the one the double search returns.  u within delta of some G_k (odds
+// about 6e-5 per comparison) or k > kSqueezeK leaves it to the double search.
+constexpr float kSqueeze = 0x1p-15f;
+constexpr int kSqueezeK = 15;
+__constant__ float kRecipF[kSqueezeK + 1] = {
+    0.0f,           0x1p+0f,        0x1p-1f,        0x1.555556p-2f, 0x1p-2f,        0x1.99999ap-3f,
+    0x1.555556p-3f, 0x1.24924ap-3f, 0x1p-3f,        0x1.c71c72p-4f, 0x1.99999ap-4f, 0x1.745d18p-4f,
+    0x1.555556p-4f, 0x1.3b13b2p-4f, 0x1.24924ap-4f, 0x1.111112p-4f};
+
+// e^-l for 0 <= l < 10 in single precision: n = rint(l / ln2), r = n ln2 - l
+// in two fma steps (ln2 split in 16 + 24 bits; n ln2_hi is exact), degree-7
+// Taylor polynomial of e^r in Horner form (|r| <= ln2/2), scaled by 2^-n.
+__device__ __forceinline__ float fexp_neg(float l) {
+    const float n = __builtin_rintf(l * 0x1.715476p+0f);
+    float r = __builtin_fmaf(n, 0x1.62e4p-1f, -l);
+    r = __builtin_fmaf(n, 0x1.7f7d1cp-20f, r);
+    float p = 0x1.a01a02p-13f;                 // 1/7!
+    p = __builtin_fmaf(p, r, 0x1.6c16c2p-10f);  // 1/6!
+    p = __builtin_fmaf(p, r, 0x1.111112p-7f);   // 1/5!
+    p = __builtin_fmaf(p, r, 0x1.555556p-5f);   // 1/4!
+    p = __builtin_fmaf(p, r, 0x1.555556p-3f);   // 1/3!
+    p = __builtin_fmaf(p, r, 0.5f);
+    p = __builtin_fmaf(p, r, 1.0f);
+    p = __builtin_fmaf(p, r, 1.0f);
+    return p * __int_as_float((127 - (int)n) << 23);
+}
+
+// The inversion result k for mean l and uniform u (both float), or -1 when
+// the squeeze cannot decide it.
+__device__ __forceinline__ int poisson_squeeze(float l, float u) {
+    float p = fexp_neg(l);
+    float G = p;
+#pragma unroll 1
+    for (int k = 0;; ++k) {
+        if (u <= G - kSqueeze) return k;
+        if (!(u > G + kSqueeze) || k == kSqueezeK) return -1;
+        p = (p * l) * kRecipF[k + 1];
+        G = G + p;
+    }
 }
 
 // Poisson(lam) from the packed word u0 (inversion) or the draw's PTRS stream.
 // Fast exit: 1 - lam <= e^-lam, and the computed exp is within 2^-51 of
 // e^-lam, so u <= (1 - lam) - 2^-50 implies u <= F_0 and the inversion
-// returns 0 -- the same value the full search gives, without the exp.
+// returns 0 -- the same value the full search gives, without the exp.  The
+// rest of the inversion range goes through the float squeeze first.
 __device__ __forceinline__ long long poisson(double lam, uint32_t u0, const DrawId& id) {
     if (!(lam > 0.0)) return 0;
-    if (lam < kPoissonInvMax && uniform01(u0) <= (1.0 - lam) - 0x1p-50) return 0;
+    if (lam < kPoissonInvMax) {
+        const double u = uniform01(u0);
+        if (u <= (1.0 - lam) - 0x1p-50) return 0;
+        const int k = poisson_squeeze((float)lam, (float)u);
+        if (k >= 0) return k;
+    }
     return poisson_slow(lam, u0, id);
+}
+
+// The fast part of poisson() for kernels that defer the rare hard cases:
+// the zero shortcut and the float squeeze settle the draw (k >= 0, the value
+// poisson() returns); -1 leaves it to the double search (inversion_search)
+// or, for mean >= 10, to PTRS.
+__device__ __forceinline__ int poisson_fast(double lam, uint32_t u0) {
+    if (!(lam > 0.0)) return 0;
+    if (!(lam < kPoissonInvMax)) return -1;
+    const double u = uniform01(u0);
+    if (u <= (1.0 - lam) - 0x1p-50) return 0;
+    return poisson_squeeze((float)lam, (float)u);
 }
 
 }  // namespace plis

@@ -16,7 +16,6 @@
 namespace plis {
 
 constexpr int kTile = 64;        // subvolumes per tile (one wavefront)
-constexpr int kGroup = 64;       // tiles per reduction group
 constexpr int kMaxS = 64;
 constexpr int kMaxR = 128;
 
@@ -44,7 +43,6 @@ struct RealInfo {
     uint64_t comp_off;   // compartment ids base (if C > 1)
     uint32_t N, id;
     uint32_t tile_begin, ntiles;
-    uint32_t group_begin, ngroups;
     uint32_t flags;      // bit0: symmetric (in-CSR aliases out-CSR), bit1: weighted
     uint32_t pad;
 };
@@ -99,14 +97,11 @@ struct KernelArgs {
     const float* iw;
     const int32_t* comp;
     PartA* partA_tile;
-    PartA* partA_grp;
     PartB* partB_tile;
-    PartB* partB_grp;
-    uint32_t* cntA_grp;
-    uint32_t* cntA_real;
-    uint32_t* cntB_grp;
-    uint32_t* cntB_real;
     uint32_t* ndone;
+    uint32_t* redo_count;    // tiles phase B left to the redo pass (reset by decide)
+    uint32_t* redo_tiles;
+    unsigned long long* redo_total;  // tiles redone since the context was made
     double epsilon;
     int32_t n_critical;
     uint32_t seed_lo, seed_hi;

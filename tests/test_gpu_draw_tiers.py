@@ -1,0 +1,172 @@
+"""The two-tier sampler of the edge-parallel leap pass.
+
+phase_b_kernel settles draws with the zero shortcut and the float squeeze,
+queues the rest per wavefront (at most 128) and draws them with the double
+search after the walk; a tile with a mean >= 10 draw or a full queue is
+redone from scratch by phase_b_redo_kernel with the complete sampler.  These
+cases drive each branch on purpose -- draws queued and drained, queues that
+overflow on some tiles only, every tile redone -- predict on the host which
+tiles the leap pass must hand to the redo pass (the same draws, rebuilt from
+the oracle's Philox and squeeze restatement), check the GPU's count against
+that, and check the trajectories bit for bit against the oracle.
+"""
+import numpy as np
+import pytest
+
+from oracle import ffi
+from pliscuda_amd import capi
+from pliscuda_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+P_REACT, P_HOP = 3, 4
+QUEUE = 128
+
+
+def _tau(net, topo, x0, eps, nc=10, seed=1):
+    tp, _, t2 = ffi.tau_all(ffi.Model(net, topo), x0, eps, nc, seed)
+    return min(tp.min(), t2.min())
+
+
+def _binom(x, r):
+    if x < r:
+        return 0.0
+    xd = float(x)
+    if r == 1:
+        return xd
+    if r == 2:
+        return (xd * (xd - 1.0)) * 0.5
+    return ((xd * (xd - 1.0)) * (xd - 2.0)) / 6.0
+
+
+def _tune_eps(net, topo, x0, lam_of_tau, lo_t, hi_t):
+    """eps whose first-step largest Poisson mean lies in [lo_t, hi_t]."""
+    lo, hi = 1e-5, 1.0
+    for _ in range(80):
+        eps = (lo * hi) ** 0.5
+        lam = lam_of_tau(_tau(net, topo, x0, eps))
+        if lam < lo_t:
+            lo = eps
+        elif lam > hi_t:
+            hi = eps
+        else:
+            return eps
+    raise AssertionError("no eps found")
+
+
+def _predict_step0(net, topo, x0, eps, seed=1, nc=10):
+    """Per tile of the first leap pass: number of draws the fast path leaves
+    undecided (mean < 10) and whether a mean >= 10 draw occurs.  Unweighted
+    symmetric topology, one compartment, realization 0, step 0, attempt 0."""
+    S, R, N = net["S"], net["R"], topo["N"]
+    x = x0.reshape(S, N).astype(np.int64)
+    d = np.asarray(net["d"], dtype=float)
+    k = np.asarray(net["k"], dtype=float)
+    reac = np.asarray(net["reac"]).reshape(S, R)
+    order = reac.sum(0)
+    tau = _tau(net, topo, x0, eps, nc, seed)
+    rp = topo["row_ptr"].astype(np.int64)
+    col = topo["col"].astype(np.int64)
+    key = (seed & 0xFFFFFFFF, seed >> 32)
+    ntiles = (N + 63) // 64
+    und = np.zeros(ntiles, dtype=np.int64)
+    big = np.zeros(ntiles, dtype=bool)
+
+    def draw(t, lam, word):
+        if not lam > 0.0:
+            return
+        if lam >= 10.0:
+            big[t] = True
+        elif ffi.squeeze_poisson(lam, word) < 0:
+            und[t] += 1
+
+    for v in range(N):
+        t = v // 64
+        for q in range(rp[v], rp[v + 1]):
+            u = col[q]
+            lo, hi = min(v, u), max(v, u)
+            dout = 1 if v > u else 0
+            for blk in range((S + 1) // 2):
+                o = ffi.philox((lo, hi, 0, (P_HOP << 7) | blk), key)
+                for s in (2 * blk, 2 * blk + 1):
+                    if s >= S:
+                        continue
+                    for src, dirbit in ((v, dout), (u, 1 - dout)):
+                        xs = x[s, src]
+                        if d[s] > 0 and xs >= nc and xs > 0:
+                            draw(t, tau * ((d[s] * float(xs)) * 1.0), o[2 * (s & 1) + dirbit])
+        o, oblk = None, -1
+        for j in range(R):
+            aj = k[j]
+            L = -1
+            for s in range(S):
+                r = int(reac[s, j])
+                if r:
+                    aj = aj * _binom(x[s, v], r)
+                    L = x[s, v] // r if L < 0 else min(L, x[s, v] // r)
+            if (order[j] > 0 and L < nc) or not aj > 0.0:
+                continue
+            if j >> 2 != oblk:
+                o, oblk = ffi.philox((v, 0xFFFFFFFF, 0, (P_REACT << 7) | (j >> 2)), key), j >> 2
+            draw(t, tau * aj, o[j & 3])
+    return und, big
+
+
+def _first_steps_match(net, topo, x0, eps, steps, expect_redo, seed=1):
+    ctx = capi.Context(net, [topo], [x0], eps=eps, seed=seed)
+    ctx.run(np.inf, step_limit=1)
+    redo = ctx.redo_tiles()
+    st = ctx.stats()
+    assert st["leapbacks"] == 0  # the prediction is for attempt 0
+    assert redo == expect_redo, (redo, expect_redo)
+    m = ffi.Model(net, topo)
+    xo = x0.copy()
+    so = ffi.run(m, xo, eps, 10, seed, np.inf, step_limit=1)
+    assert np.array_equal(ctx.get_state(), xo) and st["firings"] == so.firings
+    ctx.run(np.inf, step_limit=steps)
+    xo = x0.copy()
+    so = ffi.run(m, xo, eps, 10, seed, np.inf, step_limit=steps)
+    st = ctx.stats()
+    assert np.array_equal(ctx.get_state(), xo), "state differs"
+    for f in ("steps", "leapbacks", "firings", "subvolume_leaps"):
+        assert st[f] == getattr(so, f), f
+    assert st["t"] == so.t
+    ctx.close()
+
+
+def test_queued_draws_drained_in_tile():
+    """Conversion X0 <-> X1 plus diffusion, largest mean just under 10: every
+    tile queues a few undecided draws (hops and reactions), none overflows."""
+    net = W.network(["X0", "X1"], [({"X0": 1}, {"X1": 1}), ({"X1": 1}, {"X0": 1})], [1.0, 1.0],
+                    [1.0 / 6.0, 1.0 / 6.0])
+    topo = W.cubic_grid(8, 8, 8, periodic=True)
+    x0 = W.poisson_state(2, topo["N"], 2000.0, 11)
+    eps = _tune_eps(net, topo, x0, lambda tau: tau * float(x0.max()), 9.6, 9.95)
+    und, big = _predict_step0(net, topo, x0, eps)
+    assert not big.any() and und.min() > 0 and und.max() <= QUEUE, (und, big)
+    _first_steps_match(net, topo, x0, eps, 20, expect_redo=0)
+
+
+def test_queue_overflow_sends_some_tiles_to_redo():
+    """Eight diffusing species, hop means 8-9.7: the undecided draws of some
+    tiles exceed the queue, those tiles (and only those) are redone."""
+    S = 8
+    net = W.network([f"X{s}" for s in range(S)], [({}, {"X0": 1})], [0.0], [1.0] * S)
+    topo = W.cubic_grid(8, 8, 8, periodic=True)
+    x0 = W.poisson_state(S, topo["N"], 2000.0, 3)
+    eps = _tune_eps(net, topo, x0, lambda tau: tau * float(x0.max()), 9.6, 9.65)
+    und, big = _predict_step0(net, topo, x0, eps)
+    n_over = int((und > QUEUE).sum())
+    assert not big.any() and 0 < n_over < len(und), und
+    _first_steps_match(net, topo, x0, eps, 10, expect_redo=n_over)
+
+
+def test_large_means_redo_every_tile():
+    """Birth-death around x = 1000 with a coarse eps: means in the hundreds
+    (PTRS) in every tile, so the whole leap pass runs in the redo kernel."""
+    net = W.network(["X"], [({}, {"X": 1}), ({"X": 1}, {})], [5000.0, 5.0], [0.1])
+    topo = W.cubic_grid(8, 8, 8, periodic=True)
+    x0 = W.poisson_state(1, topo["N"], 1000.0, 5)
+    und, big = _predict_step0(net, topo, x0, 0.3)
+    assert big.all()
+    _first_steps_match(net, topo, x0, 0.3, 10, expect_redo=len(big))

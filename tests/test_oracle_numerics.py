@@ -91,6 +91,75 @@ def test_inversion_equals_exact_quantile(lam):
         assert k == int(np.searchsorted(cdf, u)), (lam, u)
 
 
+SQ_DELTA = 2.0 ** -15
+F10_BITS = int(np.array(10.0, dtype=np.float32).view(np.uint32))
+
+
+def test_squeeze_float_exp():
+    """The squeeze's float e^-l is within 4 float ulps of e^-l."""
+    L = ffi.lib()
+    for l in np.concatenate([np.geomspace(1e-30, 9.999, 4000), np.linspace(0, 9.999, 4001)]):
+        lf = float(np.float32(l))
+        g = L.orc_sq_exp_neg(lf)
+        assert abs(g - math.exp(-lf)) <= 4 * 2.0 ** -24 * math.exp(-lf), lf
+
+
+def test_squeeze_cdf_error_bound_every_float_mean():
+    """The bound the kernels' float squeeze rests on, over EVERY float mean in
+    (0, 10) and k <= 15: |G_k - F_k| < 2^-20, F_k the double search's CDF.
+    With the mean's rounding (<= 2^-21), u's rounding and G -+ delta's
+    rounding (2^-25 each) the guard band 2^-15 leaves a factor > 25."""
+    worst = ffi.squeeze_maxerr(1, F10_BITS, 1)
+    assert worst < 2.0 ** -20, worst
+    assert worst + 2.0 ** -21 + 2 * 2.0 ** -25 < SQ_DELTA / 25
+
+
+def _boundary_words(lam, n_steps=12, spread=40):
+    """Words w whose uniform (w + 0.5) 2^-32 lies within `spread` words of
+    the oracle's CDF steps F_0..F_{n_steps-1} at mean lam."""
+    L = ffi.lib()
+    p = L.orc_exp(-lam)
+    F = p
+    words = []
+    for k in range(n_steps):
+        c = int(F * 2.0 ** 32)
+        words.extend(w for w in range(c - spread, c + spread) if 0 <= w < 2**32)
+        p = (p * lam) / (k + 1)
+        F = F + p
+    return words
+
+
+@pytest.mark.parametrize("lam", [1e-6, 0.0137, 0.37, 1.0, 2.5, 6.0, 9.99])
+def test_squeeze_decisions_equal_double_search(lam):
+    """Draw by draw, the kernels' fast path (zero shortcut + float squeeze)
+    either leaves the draw to the double search or returns its result:
+    random words, and words right at each CDF step where a band-less float
+    comparison does go wrong."""
+    rng = np.random.default_rng(int(lam * 1e6) + 7)
+    words = [int(w) for w in rng.integers(0, 2**32, size=20000, dtype=np.uint64)]
+    decided = 0
+    for w in words:
+        k = ffi.squeeze_poisson(lam, w)
+        if k >= 0:
+            decided += 1
+            assert k == ffi.poisson_inversion(lam, w), (lam, w)
+    # left to the double search: k > 15, or u within the band (~6e-5 per comparison)
+    assert decided >= (stats.poisson.cdf(15, lam) - 0.003) * len(words)
+    bw = _boundary_words(lam)
+    lf = float(np.float32(lam))
+    wrong_without_band = 0
+    for w in bw:
+        exact = ffi.poisson_inversion(lam, w)
+        k = ffi.squeeze_poisson(lam, w)
+        assert k < 0 or k == exact, (lam, w)
+        uf = float(np.float32((w + 0.5) * 2.0 ** -32))
+        k0 = ffi.squeeze_decide(lf, uf, 0.0)
+        wrong_without_band += k0 >= 0 and k0 != exact
+    if lam > 1e-3:
+        # the guard band is needed: without it the float search errs here
+        assert wrong_without_band > 0, lam
+
+
 @pytest.mark.parametrize("lam", [0.05, 0.8, 4.0, 9.9, 10.0, 12.5, 40.0, 300.0, 20000.0])
 def test_poisson_goodness_of_fit(lam):
     n = 200000
