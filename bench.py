@@ -57,14 +57,15 @@ def build_workload(name):
     raise SystemExit(f"unknown workload {name}")
 
 
-def gpu_measure(wl, steps, warmup, seed=2024, autotune=True):
+def gpu_measure(wl, steps, warmup, seed=2024, autotune=True, shape=None):
     import torch
 
     from pliscuda_amd import capi
-    ctx = capi.Context(wl["net"], wl["topos"], wl["x0"], eps=wl["eps"], nc=wl["nc"], seed=seed)
-    t_end = wl["t_end"] if np.isfinite(wl["t_end"]) else float("inf")
+    wpb, bpc = shape if shape else (0, 0)
+    ctx = capi.Context(wl["net"], wl["topos"], wl["x0"], eps=wl["eps"], nc=wl["nc"], seed=seed,
+                       waves_per_block=wpb, blocks_per_cu=bpc)
     ctx.arm(t_end=float("inf"))
-    if autotune:
+    if autotune and not shape:
         ctx.autotune(3)
     shape = ctx.launch_shape()
     ctx.iterate(warmup)
@@ -204,6 +205,8 @@ def main():
     ap.add_argument("--cpu-sample-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--shape", default=None,
+                    help="WPB,BPC: fixed waves per block and blocks per CU instead of autotuning")
     ap.add_argument("--extra", action="store_true", help="also measure the Schlogl grid and batch")
     ap.add_argument("--copy-probe", action="store_true", help="only run the counter-calibration copy")
     ap.add_argument("--no-learned", action="store_true")
@@ -222,7 +225,8 @@ def main():
 
     wl = build_workload(args.workload)
     S = int(wl["net"]["S"])
-    res = gpu_measure(wl, args.steps, args.warmup, autotune=not args.no_autotune)
+    shape = tuple(int(v) for v in args.shape.split(",")) if args.shape else None
+    res = gpu_measure(wl, args.steps, args.warmup, autotune=not args.no_autotune, shape=shape)
     value = res["subvolume_leaps"] / res["seconds"]
     bytes_leap = algorithmic_bytes(S, wl["topos"], "leap")
     bytes_tau = algorithmic_bytes(S, wl["topos"], "tau")
