@@ -72,6 +72,9 @@ def _declare(L):
     L.orc_sq_poisson.restype = C.c_int
     L.orc_sq_maxerr.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
     L.orc_sq_maxerr.restype = C.c_double
+    L.orc_sq_hop_check.argtypes = [P(C.c_double), P(C.c_double), P(C.c_uint32), P(C.c_float),
+                                   P(C.c_uint32), C.c_int32, P(C.c_int32), P(C.c_int64)]
+    L.orc_sq_hop_check.restype = None
     L.orc_tau_all.argtypes = [P(Network), P(Topology), P(Params), P(C.c_uint32), C.c_uint32,
                               C.c_uint32, P(C.c_double), P(C.c_double), P(C.c_double)]
     L.orc_run.argtypes = [P(Network), P(Topology), P(Params), C.c_uint32, P(C.c_uint32),
@@ -177,6 +180,23 @@ def squeeze_poisson(lam, u0):
 
 def squeeze_decide(l, u, band):
     return int(lib().orc_sq_decide_band(l, u, band))
+
+
+def squeeze_hop_check(tau, d, x, w, u0):
+    """The kernels' single-precision hop draw (fast: k or -1) next to the
+    oracle's draw from the double mean (exact: k, or -1 for mean >= 10)."""
+    tau = np.ascontiguousarray(tau, dtype=np.float64)
+    d = np.ascontiguousarray(d, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.uint32)
+    w = np.ascontiguousarray(w, dtype=np.float32)
+    u0 = np.ascontiguousarray(u0, dtype=np.uint32)
+    n = len(tau)
+    fast = np.empty(n, dtype=np.int32)
+    exact = np.empty(n, dtype=np.int64)
+    lib().orc_sq_hop_check(_ptr(tau, C.c_double), _ptr(d, C.c_double), _ptr(x, C.c_uint32),
+                           _ptr(w, C.c_float), _ptr(u0, C.c_uint32), n, _ptr(fast, C.c_int32),
+                           _ptr(exact, C.c_int64))
+    return fast, exact
 
 
 def squeeze_maxerr(b0, b1, stride=1):

@@ -271,6 +271,28 @@ int orc_sq_poisson(double lam, uint32_t u0)
     return orc_sq_decide((float)lam, (float)u);
 }
 
+/* The kernels' fast path for a hop draw whose mean they form in single
+ * precision (tau, d, x rounded to float, w already float, three float
+ * products) and whose uniform is the word rounded to float: for each of n
+ * draws, fast[i] = k or -1 (left to the double search), exact[i] = the
+ * oracle's draw from the double mean tau ((d x) w) (-1 when that mean is
+ * >= 10, which the fast path never settles). */
+void orc_sq_hop_check(const double *tau, const double *d, const uint32_t *x, const float *w,
+                      const uint32_t *u0, int32_t n, int32_t *fast, int64_t *exact)
+{
+    static const uint32_t zc[4] = {0, 0, 0, 0}, zk[2] = {0, 0};
+    for (int32_t i = 0; i < n; i++) {
+        double lam = tau[i] * ((d[i] * (double)x[i]) * (double)w[i]);
+        float lf = (float)tau[i] * (((float)d[i] * (float)x[i]) * w[i]);
+        exact[i] = lam < POISSON_INV_MAX ? orc_poisson(lam, u0[i], zc, zk, 0) : -1;
+        if (!(lf > 0.0f)) { fast[i] = 0; continue; }
+        float u = (float)u0[i] * 0x1p-32f;
+        if (u <= (1.0f - lf) - 0x1p-19f) { fast[i] = 0; continue; }
+        if (!(lf < 9.9999f)) { fast[i] = -1; continue; }
+        fast[i] = orc_sq_decide(lf, u);
+    }
+}
+
 /* max over float means l with bit patterns b0, b0+stride, ... < b1 and over
  * k <= SQ_KMAX of |G_k(l) - F_k(l)|, F_k the double search's CDF. */
 double orc_sq_maxerr(uint32_t b0, uint32_t b1, uint32_t stride)

@@ -77,6 +77,8 @@ int orc_sq_decide(float l, float u);
 int orc_sq_decide_band(float l, float u, float band);
 int orc_sq_poisson(double lam, uint32_t u0);
 double orc_sq_maxerr(uint32_t b0, uint32_t b1, uint32_t stride);
+void orc_sq_hop_check(const double *tau, const double *d, const uint32_t *x, const float *w,
+                      const uint32_t *u0, int32_t n, int32_t *fast, int64_t *exact);
 
 /* ---- per-subvolume kernels (for kernel-level parity) ---- */
 /* tau' and a0c for every subvolume of state x (SoA [s*N+v]); tau2 uses the

@@ -289,12 +289,14 @@ static const int kVariantR[] = {4, 4, 8, 16, 128};
 template <int MS, int MR>
 static void launch_variant(plis_ctx* c, int which, const KernelArgs& a) {
     const dim3 grid(c->blocks), block(64 * c->wpb), rgrid(c->nreal), rblock(256);
+    const bool mc = c->nh.C > 1;  // several compartments: per-subvolume rate tables
     switch (which) {
         case 0:
             if constexpr (MS <= 8) {
                 if (c->learned_on) hipLaunchKernelGGL((learned_tau_kernel<MS, MR>), grid, block, 0, c->stream, a);
             }
-            hipLaunchKernelGGL((phase_a_kernel<MS, MR>), grid, block, 0, c->stream, a);
+            if (mc) hipLaunchKernelGGL((phase_a_kernel<MS, MR, true>), grid, block, 0, c->stream, a);
+            else hipLaunchKernelGGL((phase_a_kernel<MS, MR, false>), grid, block, 0, c->stream, a);
             break;
         case 1:
             hipLaunchKernelGGL((decide_kernel<MS, MR>), rgrid, rblock, 0, c->stream, a);
@@ -302,12 +304,18 @@ static void launch_variant(plis_ctx* c, int which, const KernelArgs& a) {
         case 2:
             if constexpr (MS <= 8) {
                 if (c->edge_parallel) {
-                    hipLaunchKernelGGL((phase_b_kernel<MS, MR, true>), grid, block, 0, c->stream, a);
-                    hipLaunchKernelGGL((phase_b_redo_kernel<MS, MR>), grid, block, 0, c->stream, a);
+                    if (mc) {
+                        hipLaunchKernelGGL((phase_b_kernel<MS, MR, true, true>), grid, block, 0, c->stream, a);
+                        hipLaunchKernelGGL((phase_b_redo_kernel<MS, MR, true>), grid, block, 0, c->stream, a);
+                    } else {
+                        hipLaunchKernelGGL((phase_b_kernel<MS, MR, true, false>), grid, block, 0, c->stream, a);
+                        hipLaunchKernelGGL((phase_b_redo_kernel<MS, MR, false>), grid, block, 0, c->stream, a);
+                    }
                     break;
                 }
             }
-            hipLaunchKernelGGL((phase_b_kernel<MS, MR, false>), grid, block, 0, c->stream, a);
+            if (mc) hipLaunchKernelGGL((phase_b_kernel<MS, MR, false, true>), grid, block, 0, c->stream, a);
+            else hipLaunchKernelGGL((phase_b_kernel<MS, MR, false, false>), grid, block, 0, c->stream, a);
             break;
         case 3:
             hipLaunchKernelGGL(commit_kernel, rgrid, rblock, 0, c->stream, a);

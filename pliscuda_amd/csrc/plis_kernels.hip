@@ -36,6 +36,10 @@ struct NetView {
     int S, R, C;
 };
 
+// MC = false: the network has one compartment, known at compile time, so
+// every compartment lookup folds to 0 and the rate constants and diffusion
+// constants become wave-uniform (scalar) loads.
+template <bool MC = true>
 __device__ __forceinline__ NetView net_view(const KernelArgs& a) {
     NetView n;
     n.nu = a.net_i + a.nh.off_nu;
@@ -49,7 +53,7 @@ __device__ __forceinline__ NetView net_view(const KernelArgs& a) {
     n.d = a.net_d + a.nh.off_d;
     n.S = a.nh.S;
     n.R = a.nh.R;
-    n.C = a.nh.C;
+    n.C = MC ? a.nh.C : 1;
     return n;
 }
 
@@ -362,9 +366,9 @@ __device__ void finalize_A(const KernelArgs& a, const NetView& n, const RealInfo
     a.ctl[r] = c;
 }
 
-template <int MS, int MR>
+template <int MS, int MR, bool MC>
 __global__ __launch_bounds__(256) void phase_a_kernel(KernelArgs a) {
-    const NetView n = net_view(a);
+    const NetView n = net_view<MC>(a);
     const uint32_t lane = threadIdx.x & 63u;
     for (uint32_t it = 0;; ++it) {
         const uint32_t t = tile_at(it, a.ntiles, a.waves_per_block);
@@ -667,6 +671,7 @@ __device__ __forceinline__ bool leap_subvolume(const KernelArgs& a, const NetVie
 constexpr uint32_t kQueue = 128;               // codes per wavefront
 constexpr uint32_t kQueueReact = 0x80000000u;  // reaction code: flag | lane << 8 | j
 constexpr uint32_t kQueueDefer = kQueue + 1u;  // count bump of a draw the queue does not take
+constexpr float kPtrsSureF = 10.0001f;         // a float mean above it is a double mean >= 10
 
 __device__ __forceinline__ void lds_wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -674,8 +679,10 @@ __device__ __forceinline__ void lds_wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__device__ __forceinline__ void q_append(uint32_t* s_q, uint32_t* s_qn, double lam, uint32_t code) {
-    if (!(lam < kPoissonInvMax)) {
+// A mean certainly >= 10 (PTRS) sends the tile to the redo pass; others
+// are queued (the drain re-forms the mean in double and may still defer).
+__device__ __forceinline__ void q_append(uint32_t* s_q, uint32_t* s_qn, bool ptrs, uint32_t code) {
+    if (ptrs) {
         atomicAdd(s_qn, kQueueDefer);
         return;
     }
@@ -699,6 +706,12 @@ __device__ __forceinline__ double hop_mean(double d, uint32_t x, double wq, doub
     return hop_active(d, x, nc) ? tau * ((d * (double)x) * wq) : 0.0;
 }
 
+// The same mean in single precision for the fast path: tau, d and x rounded
+// to float (w already is), three float products; within 8 2^-24 of it.
+__device__ __forceinline__ float hop_mean_f(double d, uint32_t x, float w, float tauf, int nc) {
+    return hop_active(d, x, nc) ? tauf * (((float)d * (float)x) * w) : 0.0f;
+}
+
 // One draw of the walk: complete (FULL) or fast with the queue; returns the
 // count to apply now (0 for a queued draw).
 template <bool FULL>
@@ -709,7 +722,7 @@ __device__ __forceinline__ long long walk_draw(double lam, const U4& o, const Dr
     } else {
         const int k = poisson_fast(lam, pick(o, id.word));
         if (k >= 0) return k;
-        q_append(s_q, s_qn, lam, code);
+        q_append(s_q, s_qn, !(lam < kPoissonInvMax), code);
         return 0;
     }
 }
@@ -728,6 +741,7 @@ __device__ __forceinline__ void walk_tile_draws(const KernelArgs& a, const NetVi
     const uint32_t* colp = a.col + ri.e_off;
     const float* wp = a.w + ri.e_off;
     const double tau = sv.tau;
+    const float tauf = (float)tau;
     const uint32_t k0 = a.seed_lo, k1 = a.seed_hi;
     const uint32_t c3hop = ctr_word3(sv.real, sv.attempt, kHop, 0u);
     const uint32_t eb = s_rp[0], ee = s_rp[kTile];
@@ -735,8 +749,9 @@ __device__ __forceinline__ void walk_tile_draws(const KernelArgs& a, const NetVi
         const uint32_t i = edge_owner(s_rp, e);
         const uint32_t vo = v0 + i;
         const uint32_t u = colp[e];
-        const double wq = weighted ? (double)wp[e] : 1.0;
-        const int co = s_c[i];
+        const float wf = weighted ? wp[e] : 1.0f;
+        const double wq = (double)wf;
+        const int co = n.C > 1 ? s_c[i] : 0;
         const int cu = n.C > 1 ? a.comp[ri.comp_off + u] : 0;
         const uint32_t lo = vo < u ? vo : u, hi = vo < u ? u : vo;
         const uint32_t dout = vo > u ? 1u : 0u, din = 1u - dout;
@@ -751,33 +766,49 @@ __device__ __forceinline__ void walk_tile_draws(const KernelArgs& a, const NetVi
             const uint32_t xu0 = X[ri.x_off + (uint64_t)s0 * N + u];
             const uint32_t xu1 = two ? X[ri.x_off + (uint64_t)s1 * N + u] : 0u;
             const double du0 = n.d[cu * n.S + s0], du1 = two ? n.d[cu * n.S + s1] : 0.0;
-            const double lo0 = hop_mean(do0, xo0, wq, tau, nc);
-            const double lo1 = two ? hop_mean(do1, xo1, wq, tau, nc) : 0.0;
-            const double li0 = hop_mean(du0, xu0, wq, tau, nc);
-            const double li1 = two ? hop_mean(du1, xu1, wq, tau, nc) : 0.0;
-            if (!(lo0 > 0.0 || lo1 > 0.0 || li0 > 0.0 || li1 > 0.0)) continue;
             const uint32_t blk = (uint32_t)s0 >> 1;
             const uint32_t cb = c3hop | blk;
-            const U4 o = philox10(lo, hi, sv.step, cb, k0, k1);
             const uint32_t wcode = ecode | (blk << 2);
             long long b0 = 0, b1 = 0;
-            if (lo0 > 0.0) {
-                const long long k = walk_draw<FULL>(lo0, o, DrawId{lo, hi, sv.step, cb, k0, k1, dout}, s_q, s_qn,
-                                                    wcode | dout);
-                b0 -= k;
-                fire += (unsigned long long)k;
+            if constexpr (FULL) {
+                const double lo0 = hop_mean(do0, xo0, wq, tau, nc);
+                const double lo1 = two ? hop_mean(do1, xo1, wq, tau, nc) : 0.0;
+                const double li0 = hop_mean(du0, xu0, wq, tau, nc);
+                const double li1 = two ? hop_mean(du1, xu1, wq, tau, nc) : 0.0;
+                if (!(lo0 > 0.0 || lo1 > 0.0 || li0 > 0.0 || li1 > 0.0)) continue;
+                const U4 o = philox10(lo, hi, sv.step, cb, k0, k1);
+                if (lo0 > 0.0) {
+                    const long long k = poisson(lo0, pick(o, dout), DrawId{lo, hi, sv.step, cb, k0, k1, dout});
+                    b0 -= k;
+                    fire += (unsigned long long)k;
+                }
+                if (li0 > 0.0) b0 += poisson(li0, pick(o, din), DrawId{lo, hi, sv.step, cb, k0, k1, din});
+                if (lo1 > 0.0) {
+                    const long long k =
+                        poisson(lo1, pick(o, 2u + dout), DrawId{lo, hi, sv.step, cb, k0, k1, 2u + dout});
+                    b1 -= k;
+                    fire += (unsigned long long)k;
+                }
+                if (li1 > 0.0) b1 += poisson(li1, pick(o, 2u + din), DrawId{lo, hi, sv.step, cb, k0, k1, 2u + din});
+            } else {
+                const float lo0 = hop_mean_f(do0, xo0, wf, tauf, nc);
+                const float lo1 = two ? hop_mean_f(do1, xo1, wf, tauf, nc) : 0.0f;
+                const float li0 = hop_mean_f(du0, xu0, wf, tauf, nc);
+                const float li1 = two ? hop_mean_f(du1, xu1, wf, tauf, nc) : 0.0f;
+                if (!(lo0 > 0.0f || lo1 > 0.0f || li0 > 0.0f || li1 > 0.0f)) continue;
+                const U4 o = philox10(lo, hi, sv.step, cb, k0, k1);
+                int k;
+                if ((k = poisson_fast_f(lo0, pick(o, dout))) < 0) q_append(s_q, s_qn, lo0 > kPtrsSureF, wcode | dout);
+                else { b0 -= k; fire += (unsigned long long)k; }
+                if ((k = poisson_fast_f(li0, pick(o, din))) < 0) q_append(s_q, s_qn, li0 > kPtrsSureF, wcode | din);
+                else b0 += k;
+                if ((k = poisson_fast_f(lo1, pick(o, 2u + dout))) < 0)
+                    q_append(s_q, s_qn, lo1 > kPtrsSureF, wcode | (2u + dout));
+                else { b1 -= k; fire += (unsigned long long)k; }
+                if ((k = poisson_fast_f(li1, pick(o, 2u + din))) < 0)
+                    q_append(s_q, s_qn, li1 > kPtrsSureF, wcode | (2u + din));
+                else b1 += k;
             }
-            if (li0 > 0.0)
-                b0 += walk_draw<FULL>(li0, o, DrawId{lo, hi, sv.step, cb, k0, k1, din}, s_q, s_qn, wcode | din);
-            if (lo1 > 0.0) {
-                const long long k = walk_draw<FULL>(lo1, o, DrawId{lo, hi, sv.step, cb, k0, k1, 2u + dout}, s_q,
-                                                    s_qn, wcode | (2u + dout));
-                b1 -= k;
-                fire += (unsigned long long)k;
-            }
-            if (li1 > 0.0)
-                b1 += walk_draw<FULL>(li1, o, DrawId{lo, hi, sv.step, cb, k0, k1, 2u + din}, s_q, s_qn,
-                                      wcode | (2u + din));
             if (b0 != 0) atomicAdd(&s_del[s0][i], (unsigned long long)b0);
             if (b1 != 0) atomicAdd(&s_del[s1][i], (unsigned long long)b1);
         }
@@ -786,7 +817,7 @@ __device__ __forceinline__ void walk_tile_draws(const KernelArgs& a, const NetVi
     if (v >= N) return;
     // reactions of subvolume v, propensities and criticality as react_props
     // computes them (same operations, one reaction at a time)
-    const int c = s_c[lane];
+    const int c = n.C > 1 ? s_c[lane] : 0;
     uint32_t blk = 0xFFFFFFFFu;
     U4 o{0, 0, 0, 0};
     const uint32_t c3base = ctr_word3(sv.real, sv.attempt, kReact, 0u);
@@ -819,9 +850,11 @@ __device__ __forceinline__ void walk_tile_draws(const KernelArgs& a, const NetVi
     }
 }
 
-// One queued draw (mean < 10), re-derived from its code; the double search.
+// One queued draw, re-derived from its code with its mean in double; the
+// double search.  Returns true (nothing applied) for a mean >= 10, which
+// only the redo pass draws.
 template <int MS>
-__device__ __forceinline__ void drain_draw(const KernelArgs& a, const NetView& n, const RealInfo& ri,
+__device__ __forceinline__ bool drain_draw(const KernelArgs& a, const NetView& n, const RealInfo& ri,
                                            const uint32_t* X, uint32_t v0, const StepView& sv,
                                            uint32_t (*s_x)[kTile], unsigned long long (*s_del)[kTile],
                                            const uint32_t* s_rp, const int32_t* s_c, uint32_t code,
@@ -829,18 +862,18 @@ __device__ __forceinline__ void drain_draw(const KernelArgs& a, const NetView& n
     const uint32_t k0 = a.seed_lo, k1 = a.seed_hi;
     if (code & kQueueReact) {
         const uint32_t i = (code >> 8) & 63u, j = code & 255u;
-        double aj = n.k[s_c[i] * n.R + (int)j];
+        double aj = n.k[(n.C > 1 ? s_c[i] : 0) * n.R + (int)j];
         const int nr = n.nrs[j];
         for (int q = 0; q < nr; ++q) aj = aj * binom_d(s_x[n.rs[j * 3 + q]][i], n.rr[j * 3 + q]);
         const U4 o = philox10(v0 + i, kSelfId, sv.step, ctr_word3(sv.real, sv.attempt, kReact, 0u) | (j >> 2), k0, k1);
         const long long k = inversion_search(sv.tau * aj, pick(o, j & 3u));
-        if (k == 0) return;
+        if (k == 0) return false;
         fire += (unsigned long long)k;
         for (int s = 0; s < n.S; ++s) {
             const int nu = n.nu[s * n.R + (int)j];
             if (nu != 0) atomicAdd(&s_del[s][i], (unsigned long long)(k * (long long)nu));
         }
-        return;
+        return false;
     }
     const uint32_t e = s_rp[0] + (code >> 4), w = code & 15u;
     const uint32_t blk = w >> 2, wd = w & 3u;
@@ -854,20 +887,22 @@ __device__ __forceinline__ void drain_draw(const KernelArgs& a, const NetView& n
     const bool out = (wd & 1u) == dout;
     double lam;
     if (out) {
-        lam = hop_mean(n.d[s_c[i] * n.S + s], s_x[s][i], wq, sv.tau, a.n_critical);
+        lam = hop_mean(n.d[(n.C > 1 ? s_c[i] : 0) * n.S + s], s_x[s][i], wq, sv.tau, a.n_critical);
     } else {
         const int cu = n.C > 1 ? a.comp[ri.comp_off + u] : 0;
         lam = hop_mean(n.d[cu * n.S + s], X[ri.x_off + (uint64_t)s * ri.N + u], wq, sv.tau, a.n_critical);
     }
+    if (!(lam < kPoissonInvMax)) return true;
     const U4 o = philox10(lo, hi, sv.step, ctr_word3(sv.real, sv.attempt, kHop, 0u) | blk, k0, k1);
     const long long k = inversion_search(lam, pick(o, wd));
-    if (k == 0) return;
+    if (k == 0) return false;
     if (out) {
         fire += (unsigned long long)k;
         atomicAdd(&s_del[s][i], (unsigned long long)(-k));
     } else {
         atomicAdd(&s_del[s][i], (unsigned long long)k);
     }
+    return false;
 }
 
 // The leap of one tile; false (nothing written) when the tile is left to
@@ -902,8 +937,10 @@ __device__ __forceinline__ bool leap_tile_edges(const KernelArgs& a, const NetVi
     if constexpr (!FULL) {
         const uint32_t cnt = wave_uniform(*s_qn);
         if (cnt > kQueue) return false;
+        bool ptrs = false;
         for (uint32_t q = lane; q < cnt; q += (uint32_t)kTile)
-            drain_draw<MS>(a, n, ri, X, v0, sv, s_x, s_del, s_rp, s_c, s_q[q], fire);
+            ptrs = drain_draw<MS>(a, n, ri, X, v0, sv, s_x, s_del, s_rp, s_c, s_q[q], fire) || ptrs;
+        if (__ballot(ptrs) != 0ull) return false;
         lds_wave_sync();
     }
     if (!active) return true;
@@ -1055,9 +1092,9 @@ __device__ __forceinline__ void phase_b_tile(const KernelArgs& a, const NetView&
     if (lane == 0) a.partB_tile[t] = PartB{fire, bad, viol};
 }
 
-template <int MS, int MR, bool EP>
+template <int MS, int MR, bool EP, bool MC>
 __global__ __launch_bounds__(256) void phase_b_kernel(KernelArgs a) {
-    const NetView n = net_view(a);
+    const NetView n = net_view<MC>(a);
     const uint32_t wv = wave_uniform(threadIdx.x >> 6);
     // LDS staging for the edge-parallel path (up to 4 waves per block)
     constexpr int LS = EP ? MS : 1;
@@ -1076,9 +1113,9 @@ __global__ __launch_bounds__(256) void phase_b_kernel(KernelArgs a) {
 
 // Tiles phase_b_kernel left on the redo list, drawn with the complete
 // sampler.  The list is emptied by decide_kernel before each leap pass.
-template <int MS, int MR>
+template <int MS, int MR, bool MC>
 __global__ __launch_bounds__(256) void phase_b_redo_kernel(KernelArgs a) {
-    const NetView n = net_view(a);
+    const NetView n = net_view<MC>(a);
     const uint32_t wv = wave_uniform(threadIdx.x >> 6);
     const uint32_t wpb = blockDim.x >> 6;
     __shared__ uint32_t s_x[4][MS][kTile];
@@ -1140,12 +1177,16 @@ __global__ __launch_bounds__(256) void commit_kernel(KernelArgs a) {
 // (1 species, <= 4 reactions), <= 3 x 4, <= 5 x 8 (Brusselator), <= 8 x 16
 // and the general case.
 #define PLIS_INST(MS, MR)                                                   \
-    template __global__ void phase_a_kernel<MS, MR>(KernelArgs);            \
+    template __global__ void phase_a_kernel<MS, MR, false>(KernelArgs);     \
+    template __global__ void phase_a_kernel<MS, MR, true>(KernelArgs);      \
     template __global__ void decide_kernel<MS, MR>(KernelArgs);             \
-    template __global__ void phase_b_kernel<MS, MR, false>(KernelArgs);
+    template __global__ void phase_b_kernel<MS, MR, false, false>(KernelArgs); \
+    template __global__ void phase_b_kernel<MS, MR, false, true>(KernelArgs);
 #define PLIS_INST_EP(MS, MR)                                                \
-    template __global__ void phase_b_kernel<MS, MR, true>(KernelArgs);      \
-    template __global__ void phase_b_redo_kernel<MS, MR>(KernelArgs);
+    template __global__ void phase_b_kernel<MS, MR, true, false>(KernelArgs); \
+    template __global__ void phase_b_kernel<MS, MR, true, true>(KernelArgs);  \
+    template __global__ void phase_b_redo_kernel<MS, MR, false>(KernelArgs);  \
+    template __global__ void phase_b_redo_kernel<MS, MR, true>(KernelArgs);
 PLIS_INST(1, 4)
 PLIS_INST(3, 4)
 PLIS_INST(5, 8)

@@ -277,6 +277,26 @@ __device__ __forceinline__ long long poisson(double lam, uint32_t u0, const Draw
     return poisson_slow(lam, u0, id);
 }
 
+// Largest single-precision mean the float squeeze takes: a float mean
+// below it is within 8 2^-24 (relative) of a double mean below 10, so the
+// double sampler would use the inversion search too.
+constexpr float kSqueezeMaxF = 9.9999f;
+
+// The fast part of a draw whose mean lf was formed in single precision
+// (float operands and products, within 8 2^-24 lf of the double mean lam):
+// 0 for lf <= 0; 0 when u_f <= (1 - lf) - 2^-19, which implies u <= e^-lam
+// (u_f is within 2^-25 + 2^-33 of u, the margin covers both roundings and
+// the mean's error for lf <= 1); the float squeeze below kSqueezeMaxF (its
+// guard band 2^-15 covers the mean's error, <= 5e-6 there); -1 otherwise.
+// A decided value is the one poisson() returns from the double mean.
+__device__ __forceinline__ int poisson_fast_f(float lf, uint32_t u0) {
+    if (!(lf > 0.0f)) return 0;
+    const float u = __uint2float_rn(u0) * 0x1p-32f;
+    if (u <= (1.0f - lf) - 0x1p-19f) return 0;
+    if (!(lf < kSqueezeMaxF)) return -1;
+    return poisson_squeeze(lf, u);
+}
+
 // The fast part of poisson() for kernels that defer the rare hard cases:
 // the zero shortcut and the float squeeze settle the draw (k >= 0, the value
 // poisson() returns); -1 leaves it to the double search (inversion_search)

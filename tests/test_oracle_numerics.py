@@ -160,6 +160,49 @@ def test_squeeze_decisions_equal_double_search(lam):
         assert wrong_without_band > 0, lam
 
 
+def test_float_mean_hop_draws_equal_double_search():
+    """The leap kernel forms hop means in single precision (tau, d, x to
+    float, three float products: within 8 2^-24 of the double mean) and its
+    uniform from the word rounded to float; its zero test and squeeze keep
+    margins for both.  Draw by draw over random operands (x across 2^24,
+    where x itself rounds) and over words placed at the CDF steps of each
+    mean, a settled draw equals the oracle's draw from the double mean."""
+    rng = np.random.default_rng(77)
+    n = 200000
+    tau = 10.0 ** rng.uniform(-9, 0, n)
+    d = 10.0 ** rng.uniform(-2, 1, n)
+    x = np.minimum(10.0 ** rng.uniform(0, 9.6, n), 2**32 - 1).astype(np.uint32)
+    w = rng.uniform(0.5, 1.5, n).astype(np.float32)
+    # keep means where the fast path is meant to work, mostly below 10
+    lam = tau * ((d * x.astype(float)) * w.astype(float))
+    keep = lam < 12
+    tau, d, x, w, lam = tau[keep], d[keep], x[keep], w[keep], lam[keep]
+    u0 = rng.integers(0, 2**32, size=len(tau), dtype=np.uint64).astype(np.uint32)
+    fast, exact = ffi.squeeze_hop_check(tau, d, x, w, u0)
+    ok = fast >= 0
+    assert np.array_equal(fast[ok], exact[ok])
+    assert (exact[ok] >= 0).all()  # never settles a mean >= 10
+    assert ok[lam < 5].mean() > 0.999  # above, k > 15 and means >= 10 are left to the double search
+    # words at the CDF steps of 3000 of the means (where a looser margin errs)
+    L = ffi.lib()
+    sel = rng.choice(np.flatnonzero(lam < 9.99), 3000, replace=False)
+    tt, dd, xx, ww, uu = [], [], [], [], []
+    for i in sel:
+        p = L.orc_exp(-lam[i])
+        F = p
+        for k in range(6):
+            c = int(F * 2.0**32)
+            for wd in (c - 3, c - 1, c, c + 1, c + 3):
+                if 0 <= wd < 2**32:
+                    tt.append(tau[i]); dd.append(d[i]); xx.append(x[i]); ww.append(w[i]); uu.append(wd)
+            p = (p * lam[i]) / (k + 1)
+            F = F + p
+    fast, exact = ffi.squeeze_hop_check(np.array(tt), np.array(dd), np.array(xx, dtype=np.uint32),
+                                        np.array(ww, dtype=np.float32), np.array(uu, dtype=np.uint32))
+    ok = fast >= 0
+    assert np.array_equal(fast[ok], exact[ok])
+
+
 @pytest.mark.parametrize("lam", [0.05, 0.8, 4.0, 9.9, 10.0, 12.5, 40.0, 300.0, 20000.0])
 def test_poisson_goodness_of_fit(lam):
     n = 200000
