@@ -42,6 +42,7 @@ struct plis_ctx {
     TileEnt* d_tiles = nullptr;
     uint32_t *d_x0 = nullptr, *d_x1 = nullptr;
     uint32_t *d_rp = nullptr, *d_col = nullptr, *d_irp = nullptr, *d_icol = nullptr;
+    double* d_wsum = nullptr;  // per subvolume, row_ptr layout
     float *d_w = nullptr, *d_iw = nullptr;
     int32_t* d_comp = nullptr;
     PartA* d_pa_t = nullptr;
@@ -254,6 +255,7 @@ static KernelArgs make_args(plis_ctx* c) {
     a.x0 = c->d_x0;
     a.x1 = c->d_x1;
     a.rp = c->d_rp;
+    a.wsum = c->d_wsum;
     a.col = c->d_col;
     a.w = c->d_w;
     a.irp = c->d_irp;
@@ -448,6 +450,7 @@ extern "C" int plis_create(const plis_network* net, int32_t n_real, const plis_t
     // host-side layout
     std::vector<uint32_t> hx, hrp, hcol, hirp, hicol;
     std::vector<float> hw, hiw;
+    std::vector<double> hwsum;
     std::vector<int32_t> hcomp;
     std::vector<TileEnt> tiles;
     bool any_weighted = false, any_asym = false;
@@ -478,6 +481,13 @@ extern "C" int plis_create(const plis_network* net, int32_t n_real, const plis_t
         hrp.insert(hrp.end(), g.rp.begin(), g.rp.end());
         hcol.insert(hcol.end(), g.col.begin(), g.col.end());
         hw.insert(hw.end(), g.w.begin(), g.w.end());
+        // W_v = sum of out-edge weights, summed in CSR order in double
+        for (uint32_t v = 0; v < N; ++v) {
+            double W = 0.0;
+            for (uint32_t q = g.rp[v]; q < g.rp[v + 1]; ++q) W += (double)g.w[q];
+            hwsum.push_back(W);
+        }
+        hwsum.push_back(0.0);
         if (!sym) {
             any_asym = true;
             Csr in;
@@ -540,6 +550,7 @@ extern "C" int plis_create(const plis_network* net, int32_t n_real, const plis_t
         (rc = upload(c, &c->d_info, c->info)) || (rc = upload(c, &c->d_ctl, c->ctl_host)) ||
         (rc = upload(c, &c->d_tiles, tiles)) || (rc = upload(c, &c->d_x0, hx)) ||
         (rc = dev_alloc(c, &c->d_x1, hx.size())) || (rc = upload(c, &c->d_rp, hrp)) ||
+        (rc = upload(c, &c->d_wsum, hwsum)) ||
         (rc = upload(c, &c->d_col, hcol)) ||
         (rc = upload(c, &c->d_w, any_weighted ? hw : std::vector<float>())) ||
         (rc = upload(c, &c->d_irp, hirp)) || (rc = upload(c, &c->d_icol, hicol)) ||

@@ -61,6 +61,14 @@ __device__ __forceinline__ uint32_t wave_uniform(uint32_t v) {
     return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
 }
 
+// Orders one wavefront's LDS writes before its later LDS reads.
+__device__ __forceinline__ void lds_wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+
 // XCD-aware persistent tile walk (grid is a multiple of 8 blocks).
 __device__ __forceinline__ uint32_t tile_at(uint32_t it, uint32_t ntiles, uint32_t wpb) {
     const uint32_t G = gridDim.x;
@@ -179,8 +187,7 @@ __device__ void choose_event(const KernelArgs& a, const NetView& n, const RealIn
     react_props<MR, MS>(n, x, cc, a.n_critical, av, crit);
     const uint32_t rb = a.rp[ri.rp_off + v], re = a.rp[ri.rp_off + v + 1];
     const bool weighted = (ri.flags & 2u) != 0;
-    double W = 0.0;
-    for (uint32_t q = rb; q < re; ++q) W += weighted ? (double)a.w[ri.e_off + q] : 1.0;
+    const double W = a.wsum[ri.rp_off + v];
     double a0c = 0.0;
 #pragma unroll
     for (int j = 0; j < MR; ++j)
@@ -234,38 +241,36 @@ __device__ void choose_event(const KernelArgs& a, const NetView& n, const RealIn
 // ---------------------------------------------------------------------------
 // phase A: tau' / tau'' per subvolume, min / argmin per realization
 // ---------------------------------------------------------------------------
+// Cao's tau' of one subvolume is assembled in three pieces, in this order:
+// tau_own (reactions and out-hops), tau_in_edge once per in-neighbour in
+// in-CSR order, tau_bound.  The FP64 sums therefore keep one order whether
+// the in-edges are walked by the lane itself or staged through LDS.
+template <int MS>
+struct TauAcc {
+    uint32_t x[MS];
+    double mu[MS], sig[MS];
+    double a0c;
+};
+
 template <int MS, int MR>
-__device__ __forceinline__ void tau_subvolume(const KernelArgs& a, const NetView& n, const RealInfo& ri,
-                                              const uint32_t* X, uint32_t v, double& tp_out,
-                                              double& a0c_out) {
+__device__ __forceinline__ void tau_own(const KernelArgs& a, const NetView& n, const RealInfo& ri,
+                                        const uint32_t* X, uint32_t v, int c, TauAcc<MS>& t) {
     const uint32_t N = ri.N;
     const int nc = a.n_critical;
-    uint32_t x[MS];
 #pragma unroll
-    for (int s = 0; s < MS; ++s) x[s] = s < n.S ? X[ri.x_off + (uint64_t)s * N + v] : 0u;
-    const int c = n.C > 1 ? a.comp[ri.comp_off + v] : 0;
+    for (int s = 0; s < MS; ++s) t.x[s] = s < n.S ? X[ri.x_off + (uint64_t)s * N + v] : 0u;
     double av[MR];
     uint32_t crit[(MR + 31) / 32];
-    react_props<MR, MS>(n, x, c, nc, av, crit);
-
-    const bool weighted = (ri.flags & 2u) != 0;
-    const uint32_t rb = a.rp[ri.rp_off + v], re = a.rp[ri.rp_off + v + 1];
-    double W = 0.0;
-    if (weighted) {
-        for (uint32_t q = rb; q < re; ++q) W += (double)a.w[ri.e_off + q];
-    } else {
-        W = (double)(re - rb);
-    }
-    double a0c = 0.0;
+    react_props<MR, MS>(n, t.x, c, nc, av, crit);
+    const double W = a.wsum[ri.rp_off + v];
+    t.a0c = 0.0;
 #pragma unroll
     for (int j = 0; j < MR; ++j)
-        if (j < n.R && is_crit<MR>(crit, j)) a0c += av[j];
-
-    double mu[MS], sig[MS];
+        if (j < n.R && is_crit<MR>(crit, j)) t.a0c += av[j];
 #pragma unroll
     for (int s = 0; s < MS; ++s) {
-        mu[s] = 0.0;
-        sig[s] = 0.0;
+        t.mu[s] = 0.0;
+        t.sig[s] = 0.0;
     }
 #pragma unroll
     for (int j = 0; j < MR; ++j) {
@@ -276,42 +281,41 @@ __device__ __forceinline__ void tau_subvolume(const KernelArgs& a, const NetView
             if (s >= n.S) break;
             const int nu = n.nu[s * n.R + j];
             if (nu == 0) continue;
-            mu[s] += (double)nu * av[j];
-            sig[s] += (double)(nu * nu) * av[j];
+            t.mu[s] += (double)nu * av[j];
+            t.sig[s] += (double)(nu * nu) * av[j];
         }
     }
 #pragma unroll
     for (int s = 0; s < MS; ++s) {
         if (s >= n.S) break;
         const double d = n.d[c * n.S + s];
-        if (d > 0.0 && !hop_active(d, x[s], nc)) a0c += (d * (double)x[s]) * W;
-        if (hop_active(d, x[s], nc)) {
-            const double aout = (d * (double)x[s]) * W;
-            mu[s] -= aout;
-            sig[s] += aout;
+        if (d > 0.0 && !hop_active(d, t.x[s], nc)) t.a0c += (d * (double)t.x[s]) * W;
+        if (hop_active(d, t.x[s], nc)) {
+            const double aout = (d * (double)t.x[s]) * W;
+            t.mu[s] -= aout;
+            t.sig[s] += aout;
         }
     }
-    // in-hops (nu = +1) from every in-neighbour whose hops are non-critical
-    const bool sym = (ri.flags & 1u) != 0;
-    const uint32_t* icol = sym ? a.col + ri.e_off : a.icol + ri.ie_off;
-    const float* iwp = sym ? a.w + ri.e_off : a.iw + ri.ie_off;
-    const uint32_t ib = sym ? rb : a.irp[ri.irp_off + v];
-    const uint32_t ie = sym ? re : a.irp[ri.irp_off + v + 1];
-    for (uint32_t q = ib; q < ie; ++q) {
-        const uint32_t u = icol[q];
-        const double wq = weighted ? (double)iwp[q] : 1.0;
-        const int cu = n.C > 1 ? a.comp[ri.comp_off + u] : 0;
+}
+
+// In-hops (nu = +1) from one in-neighbour, species whose hops are non-critical.
+template <int MS, typename XU>
+__device__ __forceinline__ void tau_in_edge(const NetView& n, int cu, double wq, int nc, XU xu_of,
+                                            TauAcc<MS>& t) {
 #pragma unroll
-        for (int s = 0; s < MS; ++s) {
-            if (s >= n.S) break;
-            const uint32_t xu = X[ri.x_off + (uint64_t)s * N + u];
-            const double du = n.d[cu * n.S + s];
-            if (!hop_active(du, xu, nc)) continue;
-            const double ain = (du * (double)xu) * wq;
-            mu[s] += ain;
-            sig[s] += ain;
-        }
+    for (int s = 0; s < MS; ++s) {
+        if (s >= n.S) break;
+        const uint32_t xu = xu_of(s);
+        const double du = n.d[cu * n.S + s];
+        if (!hop_active(du, xu, nc)) continue;
+        const double ain = (du * (double)xu) * wq;
+        t.mu[s] += ain;
+        t.sig[s] += ain;
     }
+}
+
+template <int MS>
+__device__ __forceinline__ double tau_bound(const KernelArgs& a, const NetView& n, const TauAcc<MS>& t) {
     double tp = kInf;
     const double eps = a.epsilon;
 #pragma unroll
@@ -319,14 +323,40 @@ __device__ __forceinline__ void tau_subvolume(const KernelArgs& a, const NetView
         if (s >= n.S) break;
         const int hor = n.hor[s];
         if (hor == 0) continue;
-        const double gg = g_cao(hor, n.mult[s], x[s]);
-        const double mm = fmax((eps * (double)x[s]) / gg, 1.0);
-        const double t1 = mm / __builtin_fabs(mu[s]);
-        const double t2 = (mm * mm) / sig[s];
+        const double gg = g_cao(hor, n.mult[s], t.x[s]);
+        const double mm = fmax((eps * (double)t.x[s]) / gg, 1.0);
+        const double t1 = mm / __builtin_fabs(t.mu[s]);
+        const double t2 = (mm * mm) / t.sig[s];
         tp = fmin(tp, fmin(t1, t2));
     }
-    tp_out = tp;
-    a0c_out = a0c;
+    return tp;
+}
+
+// tau' and the critical propensity of one subvolume, the lane walking its
+// own in-edges.
+template <int MS, int MR>
+__device__ __forceinline__ void tau_subvolume(const KernelArgs& a, const NetView& n, const RealInfo& ri,
+                                              const uint32_t* X, uint32_t v, double& tp_out,
+                                              double& a0c_out) {
+    const uint32_t N = ri.N;
+    const int c = n.C > 1 ? a.comp[ri.comp_off + v] : 0;
+    TauAcc<MS> t;
+    tau_own<MS, MR>(a, n, ri, X, v, c, t);
+    const bool weighted = (ri.flags & 2u) != 0;
+    const bool sym = (ri.flags & 1u) != 0;
+    const uint32_t* icol = sym ? a.col + ri.e_off : a.icol + ri.ie_off;
+    const float* iwp = sym ? a.w + ri.e_off : a.iw + ri.ie_off;
+    const uint32_t* irp = sym ? a.rp + ri.rp_off : a.irp + ri.irp_off;
+    const uint32_t ib = irp[v], ie = irp[v + 1];
+    for (uint32_t q = ib; q < ie; ++q) {
+        const uint32_t u = icol[q];
+        const double wq = weighted ? (double)iwp[q] : 1.0;
+        const int cu = n.C > 1 ? a.comp[ri.comp_off + u] : 0;
+        tau_in_edge<MS>(n, cu, wq, a.n_critical,
+                        [&](int s) { return X[ri.x_off + (uint64_t)s * N + u]; }, t);
+    }
+    tp_out = tau_bound<MS>(a, n, t);
+    a0c_out = t.a0c;
 }
 
 __device__ __forceinline__ void reduce_partA(double& tp, double& t2, uint32_t& v2) {
@@ -383,10 +413,9 @@ __global__ __launch_bounds__(256) void phase_a_kernel(KernelArgs a) {
         const uint32_t cur = wave_uniform(a.ctl[r].cur);
         const uint32_t* X = cur ? a.x1 : a.x0;
         const uint32_t v = te.v0 + lane;
-        double tp = kInf, t2 = kInf;
+        double tp = kInf, t2 = kInf, a0c = 0.0;
         uint32_t v2 = v;
         if (v < ri.N) {
-            double a0c;
             tau_subvolume<MS, MR>(a, n, ri, X, v, tp, a0c);
             if (a0c > 0.0) {
                 const uint32_t step = wave_uniform(a.ctl[r].step);
@@ -672,12 +701,6 @@ constexpr uint32_t kQueue = 128;               // codes per wavefront
 constexpr uint32_t kQueueReact = 0x80000000u;  // reaction code: flag | lane << 8 | j
 constexpr uint32_t kQueueDefer = kQueue + 1u;  // count bump of a draw the queue does not take
 constexpr float kPtrsSureF = 10.0001f;         // a float mean above it is a double mean >= 10
-
-__device__ __forceinline__ void lds_wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // A mean certainly >= 10 (PTRS) sends the tile to the redo pass; others
 // are queued (the drain re-forms the mean in double and may still defer).

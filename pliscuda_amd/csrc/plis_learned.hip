@@ -32,14 +32,7 @@ __device__ __forceinline__ void features_and_crit(const KernelArgs& a, const Net
     double av[MR];
     uint32_t crit[(MR + 31) / 32];
     react_props<MR, MS>(n, x, c, a.n_critical, av, crit);
-    const bool weighted = (ri.flags & 2u) != 0;
-    const uint32_t rb = a.rp[ri.rp_off + v], re = a.rp[ri.rp_off + v + 1];
-    double W = 0.0;
-    if (weighted) {
-        for (uint32_t q = rb; q < re; ++q) W += (double)a.w[ri.e_off + q];
-    } else {
-        W = (double)(re - rb);
-    }
+    const double W = a.wsum[ri.rp_off + v];
     double A[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     int maxo = 0;
     double a0c = 0.0;

@@ -90,6 +90,7 @@ struct KernelArgs {
     uint32_t* x0;
     uint32_t* x1;
     const uint32_t* rp;
+    const double* wsum;       // sum of out-edge weights per subvolume (row_ptr layout)
     const uint32_t* col;
     const float* w;
     const uint32_t* irp;
