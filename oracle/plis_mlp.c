@@ -2,7 +2,7 @@
  * plis_mlp.c -- CPU reference of the learned tau proposal (TEST
  * INFRASTRUCTURE ONLY: the checker of the gfx950 MLP kernels).
  *
- * Network 12 -> 64 -> 64 -> 2, tanh-form GELU, inputs standardised
+ * Network 12 -> 64 -> 64 -> 2, ReLU, inputs standardised
  * per feature.  Parameter blob (float, 5378 values, the first layer padded to
  * 16 inputs whose last 4 rows stay zero):
  *   W1[16][64] | b1[64] | W2[64][64] | b2[64] | W3[64][2] | b3[2]
@@ -49,15 +49,10 @@ static double bf(double x, int on)
     return (double)f;
 }
 
-static const double GC = 0.7978845608028654; /* sqrt(2/pi) */
 /* fp32 rounding of accumulator outputs, only when emulating the device */
 static double f32(double x, int on) { return on ? (double)(float)x : x; }
-static double gelu(double z) { return 0.5 * z * (1.0 + tanh(GC * (z + 0.044715 * z * z * z))); }
-static double gelu_d(double z)
-{
-    double t = tanh(GC * (z + 0.044715 * z * z * z));
-    return 0.5 * (1.0 + t) + 0.5 * z * (1.0 - t * t) * GC * (1.0 + 3.0 * 0.044715 * z * z);
-}
+static double relu(double z) { return z > 0.0 ? z : 0.0; }
+static double relu_d(double z) { return z > 0.0 ? 1.0 : 0.0; }
 
 /* norm: mu[12], sig[12]; returns per-sample outputs and (optionally) grads */
 static double mlp_pass(const float *w, const float *norm, const float *feats, int32_t n, int bfon,
@@ -74,13 +69,13 @@ static double mlp_pass(const float *w, const float *norm, const float *feats, in
             double s = 0.0;
             for (int k = 0; k < NIN; k++) s += x[k] * bf(w[P_W1 + k * NH + h], bfon);
             z1[h] = f32(s + w[P_B1 + h], bfon);
-            h1[h] = bf(f32(gelu(z1[h]), bfon), bfon);
+            h1[h] = bf(f32(relu(z1[h]), bfon), bfon);
         }
         for (int h = 0; h < NH; h++) {
             double s = 0.0;
             for (int k = 0; k < NH; k++) s += h1[k] * bf(w[P_W2 + k * NH + h], bfon);
             z2[h] = f32(s + w[P_B2 + h], bfon);
-            h2[h] = bf(f32(gelu(z2[h]), bfon), bfon);
+            h2[h] = bf(f32(relu(z2[h]), bfon), bfon);
         }
         for (int q = 0; q < NOUT; q++) {
             double s = 0.0;
@@ -102,7 +97,7 @@ static double mlp_pass(const float *w, const float *norm, const float *feats, in
         for (int k = 0; k < NH; k++) {
             double s = 0.0;
             for (int q = 0; q < NOUT; q++) s += dO[q] * bf(w[P_W3 + k * NOUT + q], bfon);
-            dz2[k] = bf(f32(s * gelu_d(z2[k]), bfon), bfon);
+            dz2[k] = bf(f32(s * relu_d(z2[k]), bfon), bfon);
         }
         for (int h = 0; h < NH; h++) {
             grad[P_B2 + h] += dz2[h];
@@ -111,7 +106,7 @@ static double mlp_pass(const float *w, const float *norm, const float *feats, in
         for (int k = 0; k < NH; k++) {
             double s = 0.0;
             for (int h = 0; h < NH; h++) s += dz2[h] * bf(w[P_W2 + k * NH + h], bfon);
-            dz1[k] = bf(f32(s * gelu_d(z1[k]), bfon), bfon);
+            dz1[k] = bf(f32(s * relu_d(z1[k]), bfon), bfon);
         }
         for (int h = 0; h < NH; h++) {
             grad[P_B1 + h] += dz1[h];

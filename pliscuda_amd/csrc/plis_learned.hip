@@ -102,7 +102,7 @@ __global__ __launch_bounds__(256) void learned_tau_kernel(KernelArgs a) {
 #pragma unroll
             for (int k = 0; k < 12; ++k) f[k] = 0.0f;
         }
-        stage_inputs(S.x, nullptr, m, f, act);
+        stage_inputs(S.h, nullptr, m, f, act, kLdH);
         float o0, o1;
         mlp_infer_tile(S, m, o0, o1);
         double tp = kInf, t2 = kInf;

@@ -1,6 +1,6 @@
 // plis_mlp.hip -- the learned tau proposal on gfx950 matrix cores.
 //
-// MLP 12 -> 64 -> 64 -> 2 (tanh-form GELU), bfloat16 operands with fp32
+// MLP 12 -> 64 -> 64 -> 2 (ReLU), bfloat16 operands with fp32
 // accumulation on v_mfma_f32_32x32x16_bf16.  One wavefront owns a tile of 64
 // rows (subvolumes or training samples); activations are staged in LDS in
 // both row- and column-major form so every MFMA fragment is a contiguous
@@ -32,7 +32,6 @@ constexpr int kQW3t = kQW2t + kNh * kNh;         // [32][64]
 constexpr int kQW2 = kQW3t + kNoPad * kNh;       // [64][64]  W2[i][o]
 constexpr int kQW3 = kQW2 + kNh * kNh;           // [64][32]  W3[i][o]
 constexpr int kQTotal = kQW3 + kNh * kNoPad;
-constexpr float kGeluC = 0.7978845608028654f;
 // LDS row strides (bf16 elements), padded against bank conflicts
 constexpr int kLdX = 24, kLdH = 72;
 
@@ -42,18 +41,8 @@ struct MlpDev {
     const float* norm;       // fmu[12] fsig[12] y_mu y_sig
 };
 
-// tanh(u) = 1 - 2 / (1 + e^{2u}) on the hardware exp / reciprocal (saturates
-// correctly for large |u|); the host reference uses the exact tanh.
-__device__ __forceinline__ float fast_tanh(float u) {
-    return 1.0f - 2.0f * __frcp_rn(1.0f + __expf(2.0f * u));
-}
-__device__ __forceinline__ float gelu_f(float z) {
-    return 0.5f * z * (1.0f + fast_tanh(kGeluC * (z + 0.044715f * z * z * z)));
-}
-__device__ __forceinline__ float gelu_d(float z) {
-    const float t = fast_tanh(kGeluC * (z + 0.044715f * z * z * z));
-    return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * kGeluC * (1.0f + 3.0f * 0.044715f * z * z);
-}
+__device__ __forceinline__ float act_f(float z) { return fmaxf(z, 0.0f); }
+__device__ __forceinline__ float act_d(float z) { return z > 0.0f ? 1.0f : 0.0f; }
 
 // lane-layout of v_mfma_f32_32x32x16_bf16: A fragment = 8 consecutive k of
 // row (l % 32) starting at k = 8*(l/32); B fragment = 8 consecutive k of
@@ -79,7 +68,7 @@ struct MlpLds {
     __bf16 h1[64 * kLdH], h1t[64 * kLdH];
     __bf16 h2[64 * kLdH], h2t[64 * kLdH];
     __bf16 dz[64 * kLdH], dzt[64 * kLdH];
-    float g1[64 * 65], g2[64 * 65];  // gelu'(z) [row][col]
+    float g1[64 * 65], g2[64 * 65];  // relu'(z) [row][col]
     float out[64 * 2];
 };
 
@@ -89,30 +78,30 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// LDS of the inference-only path (one wavefront): one activation buffer is
-// reused by both hidden layers.
+// LDS of the inference-only path (one wavefront): one buffer of 64 rows
+// holds the standardised inputs (columns 0..15) and then each hidden layer's
+// activations, every layer reading a tile half before overwriting it.
 struct MlpLdsInfer {
-    __bf16 x[64 * kLdX];
     __bf16 h[64 * kLdH];
 };
 
-// Standardise lane's feature row into LDS (lane r owns row r).
+// Standardise lane's feature row into LDS row r (stride ld; lane r owns row r).
 __device__ __forceinline__ void stage_inputs(__bf16* x, __bf16* xt, const MlpDev& m, const float (&f)[12],
-                                             bool valid) {
+                                             bool valid, int ld = kLdX) {
     const int r = threadIdx.x & 63;
 #pragma unroll
     for (int k = 0; k < kNin; ++k) {
         float z = 0.0f;
-        if (k < 12 && valid) z = (f[k] - m.norm[k]) / m.norm[12 + k];
+        if (k < 12 && valid) z = (f[k] - m.norm[k]) * (1.0f / m.norm[12 + k]);
         const __bf16 b = (__bf16)z;
-        x[r * kLdX + k] = b;
+        x[r * ld + k] = b;
         if (xt) xt[k * kLdH + r] = b;
     }
 }
 
 // One dense layer on the 64-row tile: out[64][N] = act(in[64][K] . W + b).
-// Writes bf16 activations (row-major and transposed) and gelu'(z).
-template <int K, int NB, bool GELU>
+// Writes bf16 activations (row-major and transposed) and relu'(z).
+template <int K, int NB, bool ACT>
 __device__ __forceinline__ void dense(const __bf16* in, int ldin, const __bf16* wt, const float* bias,
                                       __bf16* h, __bf16* ht, float* gd, float* out2) {
 #pragma unroll
@@ -127,12 +116,12 @@ __device__ __forceinline__ void dense(const __bf16* in, int ldin, const __bf16* 
 #pragma unroll
             for (int i = 0; i < 16; ++i) {
                 const int row = mb * 32 + acc_row(i);
-                if (GELU) {
+                if (ACT) {
                     const float z = c[i] + bias[col];
-                    const __bf16 a = (__bf16)gelu_f(z);
+                    const __bf16 a = (__bf16)act_f(z);
                     h[row * kLdH + col] = a;
                     if (ht) ht[col * kLdH + row] = a;
-                    if (gd) gd[row * 65 + col] = gelu_d(z);
+                    if (gd) gd[row * 65 + col] = act_d(z);
                 } else if (col < kNo) {
                     out2[row * 2 + col] = c[i] + bias[col];
                 }
@@ -141,7 +130,7 @@ __device__ __forceinline__ void dense(const __bf16* in, int ldin, const __bf16* 
 }
 
 // Forward pass of the tile; returns the lane's two outputs.  Transposed
-// activations and gelu'(z) are kept only when h1t/h2t/g1/g2 are given
+// activations and relu'(z) are kept only when h1t/h2t/g1/g2 are given
 // (training).
 __device__ __forceinline__ void mlp_forward_tile(const __bf16* x, __bf16* h1, __bf16* h1t, __bf16* h2,
                                                  __bf16* h2t, float* g1, float* g2, float* out,
@@ -222,7 +211,7 @@ __global__ __launch_bounds__(64) void mlp_train_kernel(TrainArgs a) {
                 dw3[mb] = mfma(lds_frag(s.h2t, kLdH, mb * 32, kb * 16), lds_frag(s.dzt, kLdH, 0, kb * 16), dw3[mb]);
         if (l < kNo)
             for (int r = 0; r < 64; ++r) db3 += (float)s.dz[r * kLdH + l];
-        // dH2 = dO . W3^T -> dZ2 = dH2 * gelu'(Z2)
+        // dH2 = dO . W3^T -> dZ2 = dH2 * relu'(Z2)
         f32x16 dh[4];
 #pragma unroll
         for (int mb = 0; mb < 2; ++mb)
@@ -380,44 +369,74 @@ __global__ __launch_bounds__(256) void mlp_pack_kernel(const float* w32, __bf16*
     q[e] = (__bf16)v;
 }
 
-// Inference forward of the tile (one wavefront): layers 1-2 on MFMA with one
-// shared LDS buffer (layer 2 reads all its A fragments before overwriting),
-// the 64 -> 2 output layer on the VALU from the lane's own row.
-__device__ __forceinline__ void mlp_infer_tile(MlpLdsInfer& s, const MlpDev& m, float& o0, float& o1) {
-    wave_sync();
-    dense<16, 2, true>(s.x, kLdX, m.q + kQW1t, m.w32 + kPB1, s.h, nullptr, nullptr, nullptr);
-    wave_sync();
-    bf16x8 af[2][4];
+// One hidden layer of the inference tile in transposed form,
+// out^T = relu(W^T in^T + b): the A operand is the packed W^T (rows = units),
+// the B operand the tile rows of `in`, so lane l's accumulator holds four
+// consecutive units of one tile row and stores them with one 8-byte write
+// into the row-major activations the next layer reads.  The two halves of
+// the tile (rows 0..31, 32..63) go one after the other; a half's B
+// fragments are read before its rows are written, so `out` may be `in`.
+template <int K>
+__device__ __forceinline__ void infer_layer(const __bf16* in, int ldin, const __bf16* wt, const float* bias,
+                                            __bf16* out) {
+    typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+    const int l = threadIdx.x & 63;
+#pragma unroll 1
+    for (int nb = 0; nb < 2; ++nb) {
+        bf16x8 bfr[K / 16];
 #pragma unroll
-    for (int mb = 0; mb < 2; ++mb)
-#pragma unroll
-        for (int kb = 0; kb < 4; ++kb) af[mb][kb] = lds_frag(s.h, kLdH, mb * 32, kb * 16);
-    wave_sync();
-#pragma unroll
-    for (int mb = 0; mb < 2; ++mb)
-#pragma unroll
-        for (int nb = 0; nb < 2; ++nb) {
+        for (int kb = 0; kb < K / 16; ++kb) bfr[kb] = lds_frag(in, ldin, nb * 32, kb * 16);
+        wave_sync();
+        const int row = nb * 32 + (l & 31);
+#pragma unroll 1
+        for (int mb = 0; mb < 2; ++mb) {
             f32x16 c = {};
 #pragma unroll
-            for (int kb = 0; kb < 4; ++kb) c = mfma(af[mb][kb], glb_frag(m.q + kQW2t, kNh, nb * 32, kb * 16), c);
-            const int col = nb * 32 + acc_col();
+            for (int kb = 0; kb < K / 16; ++kb) c = mfma(glb_frag(wt, K, mb * 32, kb * 16), bfr[kb], c);
 #pragma unroll
-            for (int i = 0; i < 16; ++i) {
-                const int row = mb * 32 + acc_row(i);
-                s.h[row * kLdH + col] = (__bf16)gelu_f(c[i] + m.w32[kPB2 + col]);
+            for (int j = 0; j < 4; ++j) {
+                const int u0 = mb * 32 + 8 * j + 4 * (l >> 5);
+                const float4 bb = *reinterpret_cast<const float4*>(bias + u0);
+                bf16x4 v;
+                v[0] = (__bf16)act_f(c[4 * j + 0] + bb.x);
+                v[1] = (__bf16)act_f(c[4 * j + 1] + bb.y);
+                v[2] = (__bf16)act_f(c[4 * j + 2] + bb.z);
+                v[3] = (__bf16)act_f(c[4 * j + 3] + bb.w);
+                *reinterpret_cast<bf16x4*>(out + row * kLdH + u0) = v;
             }
         }
-    wave_sync();
-    const int r = threadIdx.x & 63;
-    float a0 = 0.0f, a1 = 0.0f;
-#pragma unroll 8
-    for (int k = 0; k < kNh; ++k) {
-        const float h = (float)s.h[r * kLdH + k];
-        a0 = fmaf(h, (float)m.q[kQW3 + k * kNoPad + 0], a0);
-        a1 = fmaf(h, (float)m.q[kQW3 + k * kNoPad + 1], a1);
+        wave_sync();
     }
-    o0 = a0 + m.w32[kPB3 + 0];
-    o1 = a1 + m.w32[kPB3 + 1];
+}
+
+// Forward pass of one 64-row tile for inference (inputs staged in s.h);
+// returns lane r's two outputs (row r).  The output layer is W3^T (padded
+// to 32 rows) times the tile: lanes 0..31 receive rows l and 32 + l.
+__device__ __forceinline__ void mlp_infer_tile(MlpLdsInfer& s, const MlpDev& m, float& o0, float& o1) {
+    wave_sync();
+    infer_layer<kNin>(s.h, kLdH, m.q + kQW1t, m.w32 + kPB1, s.h);
+    wave_sync();
+    infer_layer<kNh>(s.h, kLdH, m.q + kQW2t, m.w32 + kPB2, s.h);
+    wave_sync();
+    const int l = threadIdx.x & 63;
+    float a0 = 0.0f, a1 = 0.0f, b0 = 0.0f, b1 = 0.0f;
+#pragma unroll 1
+    for (int nb = 0; nb < 2; ++nb) {
+        f32x16 c = {};
+#pragma unroll
+        for (int kb = 0; kb < kNh / 16; ++kb)
+            c = mfma(glb_frag(m.q + kQW3t, kNh, 0, kb * 16), lds_frag(s.h, kLdH, nb * 32, kb * 16), c);
+        if (nb == 0) {
+            a0 = c[0];
+            a1 = c[1];
+        } else {
+            b0 = c[0];
+            b1 = c[1];
+        }
+    }
+    const float t0 = __shfl(b0, (l + 32) & 63), t1 = __shfl(b1, (l + 32) & 63);
+    o0 = (l < 32 ? a0 : t0) + m.w32[kPB3 + 0];
+    o1 = (l < 32 ? a1 : t1) + m.w32[kPB3 + 1];
 }
 
 // Inference over given feature rows (parity checks): out[n][2].
@@ -429,7 +448,7 @@ __global__ __launch_bounds__(64) void mlp_infer_kernel(MlpDev m, const float* fe
         float f[12];
 #pragma unroll
         for (int k = 0; k < 12; ++k) f[k] = r < n ? feats[(size_t)r * 12 + k] : 0.f;
-        stage_inputs(s.x, nullptr, m, f, r < n);
+        stage_inputs(s.h, nullptr, m, f, r < n, kLdH);
         float o0, o1;
         mlp_infer_tile(s, m, o0, o1);
         if (r < n) {
