@@ -290,7 +290,7 @@ static const int kVariantR[] = {4, 4, 8, 16, 128};
 // pass, 3 commit; 5 = training records of one realization.
 template <int MS, int MR>
 static void launch_variant(plis_ctx* c, int which, const KernelArgs& a) {
-    const dim3 grid(c->blocks), block(64 * c->wpb), rgrid(c->nreal), rblock(256);
+    const dim3 grid(c->blocks), block(64 * c->wpb), rgrid(c->nreal), rblock(1024), dblock(decide_threads<MS>());
     const bool mc = c->nh.C > 1;  // several compartments: per-subvolume rate tables
     switch (which) {
         case 0:
@@ -301,7 +301,7 @@ static void launch_variant(plis_ctx* c, int which, const KernelArgs& a) {
             else hipLaunchKernelGGL((phase_a_kernel<MS, MR, false>), grid, block, 0, c->stream, a);
             break;
         case 1:
-            hipLaunchKernelGGL((decide_kernel<MS, MR>), rgrid, rblock, 0, c->stream, a);
+            hipLaunchKernelGGL((decide_kernel<MS, MR>), rgrid, dblock, 0, c->stream, a);
             break;
         case 2:
             if constexpr (MS <= 8) {
@@ -351,7 +351,7 @@ static void set_shape(plis_ctx* c, uint32_t wpb, uint32_t bpc) {
     blocks = std::max(8u, (blocks + 7u) & ~7u);
     c->blocks = blocks;
     if (c->graph) {
-        hipGraphExecDestroy(c->graph);
+        (void)hipGraphExecDestroy(c->graph);
         c->graph = nullptr;
         c->graph_iters = 0;
     }
@@ -368,7 +368,7 @@ static int graph_iters(plis_ctx* c, uint32_t n) {
     if (!c->use_graph) return launch_iters(c, n);
     if (c->graph_iters != n) {
         if (c->graph) {
-            hipGraphExecDestroy(c->graph);
+            (void)hipGraphExecDestroy(c->graph);
             c->graph = nullptr;
         }
         hipGraph_t g;
@@ -406,11 +406,12 @@ extern "C" const char* plis_last_error(const plis_ctx* ctx) {
 
 extern "C" void plis_destroy(plis_ctx* c) {
     if (!c) return;
-    if (c->stream) hipStreamSynchronize(c->stream);
-    if (c->graph) hipGraphExecDestroy(c->graph);
-    for (void* p : c->allocs) hipFree(p);
-    if (c->h_ndone) hipHostFree(c->h_ndone);
-    if (c->stream) hipStreamDestroy(c->stream);
+    // teardown: release everything even if an earlier call failed
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->graph) (void)hipGraphExecDestroy(c->graph);
+    for (void* p : c->allocs) (void)hipFree(p);
+    if (c->h_ndone) (void)hipHostFree(c->h_ndone);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
@@ -526,7 +527,7 @@ extern "C" int plis_create(const plis_network* net, int32_t n_real, const plis_t
     c->ntiles = tile;
     c->total_counts = xoff;
 
-    hipSetDevice(0);
+    if (hipSetDevice(0) != hipSuccess) { delete c; g_create_err = "hipSetDevice failed"; return PLIS_ERR_NODEV; }
     c->device = 0;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, 0) != hipSuccess) { delete c; g_create_err = "hipGetDeviceProperties failed"; return PLIS_ERR_HIP; }
@@ -761,9 +762,9 @@ extern "C" int plis_debug_tau(plis_ctx* c, double* tp, double* a0c, double* t2) 
     if (e == hipSuccess) e = hipMemcpy(tp, d_tp, n * sizeof(double), hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(a0c, d_a0c, n * sizeof(double), hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(t2, d_t2, n * sizeof(double), hipMemcpyDeviceToHost);
-    hipFree(d_tp);
-    hipFree(d_a0c);
-    hipFree(d_t2);
+    (void)hipFree(d_tp);
+    (void)hipFree(d_a0c);
+    (void)hipFree(d_t2);
     if (e != hipSuccess) {
         c->err = std::string("plis_debug_tau: ") + hipGetErrorString(e);
         return PLIS_ERR_HIP;

@@ -439,15 +439,20 @@ __global__ __launch_bounds__(256) void phase_a_kernel(KernelArgs a) {
 // One block per realization: min tau' / argmin tau'' over the realization's
 // tile partials (written by phase A; the kernel boundary makes them visible),
 // then the step decision.
+// Threads of decide_kernel: wide blocks read the tile partials in fewer
+// rounds; the large variants stay at 256 to keep their registers.
+template <int MS>
+constexpr uint32_t decide_threads() { return MS <= 5 ? 1024u : 256u; }
+
 template <int MS, int MR>
-__global__ __launch_bounds__(256) void decide_kernel(KernelArgs a) {
+__global__ __launch_bounds__(decide_threads<MS>()) void decide_kernel(KernelArgs a) {
     const uint32_t r = blockIdx.x;
     if (r == 0 && threadIdx.x == 0) *a.redo_count = 0u;  // the coming leap pass's redo list
     if (r >= a.nreal) return;
     const RealInfo ri = a.info[r];
     if (a.ctl[r].done || a.ctl[r].retry) return;
-    __shared__ double s_tp[4], s_t2[4];
-    __shared__ uint32_t s_v2[4];
+    __shared__ double s_tp[16], s_t2[16];
+    __shared__ uint32_t s_v2[16];
     double tp = kInf, t2 = kInf;
     uint32_t v2 = 0xFFFFFFFFu;
     // 8 independent loads in flight per thread
@@ -1155,13 +1160,13 @@ __global__ __launch_bounds__(256) void phase_b_redo_kernel(KernelArgs a) {
 
 // One block per realization: sum of firings / any negative over the tile
 // partials of phase B, then commit the step or leap back.
-__global__ __launch_bounds__(256) void commit_kernel(KernelArgs a) {
+__global__ __launch_bounds__(1024) void commit_kernel(KernelArgs a) {
     const uint32_t r = blockIdx.x;
     if (r >= a.nreal) return;
     const RealInfo ri = a.info[r];
     if (a.ctl[r].done) return;
-    __shared__ unsigned long long s_f[4];
-    __shared__ uint32_t s_b[4], s_v[4];
+    __shared__ unsigned long long s_f[16];
+    __shared__ uint32_t s_b[16], s_v[16];
     unsigned long long fire = 0;
     uint32_t bad = 0, viol = 0;
     for (uint32_t q0 = 0; q0 < ri.ntiles; q0 += 8u * blockDim.x) {
