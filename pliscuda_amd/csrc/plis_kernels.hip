@@ -23,6 +23,8 @@
 // 256-byte access.  Blocks are dispatched round-robin over the 8 XCDs, so
 // each XCD walks a contiguous eighth of the tile list (its L2 holds the
 // neighbouring tiles it gathers from).
+#include <type_traits>
+
 #include "plis_numerics.h"
 #include "plis_types.h"
 
@@ -773,6 +775,9 @@ __device__ __forceinline__ void walk_tile_draws(const KernelArgs& a, const NetVi
     const uint32_t k0 = a.seed_lo, k1 = a.seed_hi;
     const uint32_t c3hop = ctr_word3(sv.real, sv.attempt, kHop, 0u);
     const uint32_t eb = s_rp[0], ee = s_rp[kTile];
+    // fast-path hop firings of this lane: each settled draw is <= 15 and a
+    // tile has < 2^27 edges, so 32 bits hold them
+    uint32_t fire_fast = 0u;
     for (uint32_t e = eb + lane; e < ee; e += (uint32_t)kTile) {
         const uint32_t i = edge_owner(s_rp, e);
         const uint32_t vo = v0 + i;
@@ -797,7 +802,7 @@ __device__ __forceinline__ void walk_tile_draws(const KernelArgs& a, const NetVi
             const uint32_t blk = (uint32_t)s0 >> 1;
             const uint32_t cb = c3hop | blk;
             const uint32_t wcode = ecode | (blk << 2);
-            long long b0 = 0, b1 = 0;
+            std::conditional_t<FULL, long long, int> b0 = 0, b1 = 0;
             if constexpr (FULL) {
                 const double lo0 = hop_mean(do0, xo0, wq, tau, nc);
                 const double lo1 = two ? hop_mean(do1, xo1, wq, tau, nc) : 0.0;
@@ -827,20 +832,21 @@ __device__ __forceinline__ void walk_tile_draws(const KernelArgs& a, const NetVi
                 const U4 o = philox10(lo, hi, sv.step, cb, k0, k1);
                 int k;
                 if ((k = poisson_fast_f(lo0, pick(o, dout))) < 0) q_append(s_q, s_qn, lo0 > kPtrsSureF, wcode | dout);
-                else { b0 -= k; fire += (unsigned long long)k; }
+                else { b0 -= k; fire_fast += (uint32_t)k; }
                 if ((k = poisson_fast_f(li0, pick(o, din))) < 0) q_append(s_q, s_qn, li0 > kPtrsSureF, wcode | din);
                 else b0 += k;
                 if ((k = poisson_fast_f(lo1, pick(o, 2u + dout))) < 0)
                     q_append(s_q, s_qn, lo1 > kPtrsSureF, wcode | (2u + dout));
-                else { b1 -= k; fire += (unsigned long long)k; }
+                else { b1 -= k; fire_fast += (uint32_t)k; }
                 if ((k = poisson_fast_f(li1, pick(o, 2u + din))) < 0)
                     q_append(s_q, s_qn, li1 > kPtrsSureF, wcode | (2u + din));
                 else b1 += k;
             }
-            if (b0 != 0) atomicAdd(&s_del[s0][i], (unsigned long long)b0);
-            if (b1 != 0) atomicAdd(&s_del[s1][i], (unsigned long long)b1);
+            if (b0 != 0) atomicAdd(&s_del[s0][i], (unsigned long long)(long long)b0);
+            if (b1 != 0) atomicAdd(&s_del[s1][i], (unsigned long long)(long long)b1);
         }
     }
+    fire += fire_fast;
     const uint32_t v = v0 + lane;
     if (v >= N) return;
     // reactions of subvolume v, propensities and criticality as react_props
@@ -1120,8 +1126,11 @@ __device__ __forceinline__ void phase_b_tile(const KernelArgs& a, const NetView&
     if (lane == 0) a.partB_tile[t] = PartB{fire, bad, viol};
 }
 
+// The one-compartment edge-parallel leap of up to 5 species fits in 96
+// VGPRs without spills; asking for 5 waves per SIMD keeps it there.
 template <int MS, int MR, bool EP, bool MC>
-__global__ __launch_bounds__(256) void phase_b_kernel(KernelArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EP && !MC && MS <= 5 ? 5 : 1)))
+void phase_b_kernel(KernelArgs a) {
     const NetView n = net_view<MC>(a);
     const uint32_t wv = wave_uniform(threadIdx.x >> 6);
     // LDS staging for the edge-parallel path (up to 4 waves per block)
