@@ -141,6 +141,17 @@ int plis_debug_tau(plis_ctx *ctx, double *tau_prime, double *a0c, double *tau2);
 /* Launch shape in use: out[0] waves per block, out[1] blocks, out[2] tiles. */
 int plis_launch_shape(plis_ctx *ctx, int32_t *out);
 
+/* Checkpoint / resume.  No counterpart in the reference, whose snapshots
+ * hold counts only (PLISCuda/src/driver.cu:362-375); the blob holds every
+ * realization's counts, clock, step and attempt indices, tau state and
+ * statistics, which with counter-based draws fix the rest of the trajectory
+ * bit for bit.  plis_restore needs a context created from the same network,
+ * topologies, realization ids, epsilon, n_c and seed; it fails with
+ * PLIS_ERR_ARG otherwise.  The learned network is saved with plis_mlp_get. */
+int plis_checkpoint_size(plis_ctx *ctx, uint64_t *bytes);
+int plis_checkpoint(plis_ctx *ctx, void *buf, uint64_t bytes);
+int plis_restore(plis_ctx *ctx, const void *buf, uint64_t bytes);
+
 /* Diagnostic (no counterpart in the reference): number of tiles the
  * edge-parallel leap passes of this context have handed to their redo pass
  * since plis_create, i.e. tiles holding a Poisson draw of mean >= 10 or more

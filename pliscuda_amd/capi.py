@@ -20,7 +20,7 @@ ERRORS = {-1: "PLIS_ERR_ARG", -2: "PLIS_ERR_HIP", -3: "PLIS_ERR_LEAPBACK", -4: "
 # every entry point declared in include/plis.h
 EXPORTS = ("plis_create", "plis_run", "plis_run_until", "plis_arm", "plis_iterate", "plis_profile",
            "plis_set_state", "plis_get_state", "plis_get_stats", "plis_sync", "plis_debug_tau",
-           "plis_launch_shape", "plis_redo_tiles", "plis_autotune", "plis_destroy", "plis_last_error",
+           "plis_launch_shape", "plis_redo_tiles", "plis_checkpoint_size", "plis_checkpoint", "plis_restore", "plis_autotune", "plis_destroy", "plis_last_error",
            "plis_device_count", "plis_mlp_set", "plis_mlp_get", "plis_mlp_forward",
            "plis_mlp_loss_grad", "plis_mlp_train", "plis_records", "plis_set_tau_mode")
 MLP_PARAMS = 5378
@@ -85,6 +85,9 @@ def _declare(L):
     L.plis_debug_tau.argtypes = [vp, P(C.c_double), P(C.c_double), P(C.c_double)]
     L.plis_launch_shape.argtypes = [vp, P(C.c_int32)]
     L.plis_redo_tiles.argtypes = [vp, P(C.c_uint64)]
+    L.plis_checkpoint_size.argtypes = [vp, P(C.c_uint64)]
+    L.plis_checkpoint.argtypes = [vp, C.c_void_p, C.c_uint64]
+    L.plis_restore.argtypes = [vp, C.c_void_p, C.c_uint64]
     L.plis_autotune.argtypes = [vp, C.c_uint32]
     L.plis_destroy.argtypes = [vp]
     L.plis_destroy.restype = None
@@ -200,6 +203,19 @@ class Context:
         out = (C.c_int32 * 3)()
         self._chk(lib().plis_launch_shape(self.h, out), "plis_launch_shape")
         return {"waves_per_block": out[0], "blocks": out[1], "tiles": out[2]}
+
+    def checkpoint(self):
+        """Every realization's counts, clock, indices and statistics as bytes."""
+        n = C.c_uint64()
+        self._chk(lib().plis_checkpoint_size(self.h, C.byref(n)), "plis_checkpoint_size")
+        buf = C.create_string_buffer(n.value)
+        self._chk(lib().plis_checkpoint(self.h, buf, n.value), "plis_checkpoint")
+        return buf.raw
+
+    def restore(self, blob):
+        """Continue from a checkpoint of a context with the same model and seed."""
+        buf = C.create_string_buffer(bytes(blob), len(blob))
+        self._chk(lib().plis_restore(self.h, buf, len(blob)), "plis_restore")
 
     def redo_tiles(self):
         """Tiles the edge-parallel leap passes have left to their redo pass

@@ -722,6 +722,108 @@ extern "C" int plis_get_state(plis_ctx* c, int32_t r, uint32_t* x) {
     return PLIS_OK;
 }
 
+// ---- checkpoint / resume ----
+// Blob: CkptHeader | N[nreal] | id[nreal] | RealCtl[nreal] | counts (each
+// realization's current buffer, species-major, realizations in order).
+struct CkptHeader {
+    char magic[8];
+    uint32_t version, nreal;
+    int32_t S, R;
+    uint64_t total_counts, seed;
+    double eps;
+    int32_t nc;
+    uint32_t ctl_bytes;
+};
+constexpr char kCkptMagic[8] = {'P', 'L', 'I', 'S', 'C', 'K', 'P', 'T'};
+
+static uint64_t ckpt_bytes(const plis_ctx* c) {
+    return sizeof(CkptHeader) + 2ull * c->nreal * sizeof(uint32_t) + (uint64_t)c->nreal * sizeof(RealCtl) +
+           c->total_counts * sizeof(uint32_t);
+}
+
+extern "C" int plis_checkpoint_size(plis_ctx* c, uint64_t* bytes) {
+    if (!c || !bytes) return PLIS_ERR_ARG;
+    *bytes = ckpt_bytes(c);
+    return PLIS_OK;
+}
+
+extern "C" int plis_checkpoint(plis_ctx* c, void* buf, uint64_t bytes) {
+    if (!c || !buf) return PLIS_ERR_ARG;
+    if (bytes < ckpt_bytes(c)) {
+        c->err = "plis_checkpoint: buffer smaller than plis_checkpoint_size";
+        return PLIS_ERR_ARG;
+    }
+    if (int rc = pull_ctl(c)) return rc;
+    char* p = static_cast<char*>(buf);
+    CkptHeader h{};
+    std::memcpy(h.magic, kCkptMagic, sizeof h.magic);
+    h.version = 1u;
+    h.nreal = c->nreal;
+    h.S = c->S;
+    h.R = c->R;
+    h.total_counts = c->total_counts;
+    h.seed = c->seed;
+    h.eps = c->eps;
+    h.nc = c->nc;
+    h.ctl_bytes = (uint32_t)sizeof(RealCtl);
+    std::memcpy(p, &h, sizeof h);
+    p += sizeof h;
+    for (uint32_t r = 0; r < c->nreal; ++r) std::memcpy(p + 4ull * r, &c->info[r].N, 4);
+    p += 4ull * c->nreal;
+    for (uint32_t r = 0; r < c->nreal; ++r) std::memcpy(p + 4ull * r, &c->info[r].id, 4);
+    p += 4ull * c->nreal;
+    std::memcpy(p, c->ctl_host.data(), c->nreal * sizeof(RealCtl));
+    p += c->nreal * sizeof(RealCtl);
+    for (uint32_t r = 0; r < c->nreal; ++r) {
+        const RealInfo& ri = c->info[r];
+        const uint32_t* X = c->ctl_host[r].cur ? c->d_x1 : c->d_x0;
+        HIPCHK(c, hipMemcpy(p + 4ull * ri.x_off, X + ri.x_off, (size_t)c->S * ri.N * sizeof(uint32_t),
+                            hipMemcpyDeviceToHost));
+    }
+    return PLIS_OK;
+}
+
+extern "C" int plis_restore(plis_ctx* c, const void* buf, uint64_t bytes) {
+    if (!c || !buf) return PLIS_ERR_ARG;
+    const char* p = static_cast<const char*>(buf);
+    CkptHeader h;
+    if (bytes < sizeof h) {
+        c->err = "plis_restore: blob too short";
+        return PLIS_ERR_ARG;
+    }
+    std::memcpy(&h, p, sizeof h);
+    if (std::memcmp(h.magic, kCkptMagic, sizeof h.magic) != 0 || h.version != 1u ||
+        h.ctl_bytes != sizeof(RealCtl)) {
+        c->err = "plis_restore: not a checkpoint of this library version";
+        return PLIS_ERR_ARG;
+    }
+    if (h.nreal != c->nreal || h.S != c->S || h.R != c->R || h.total_counts != c->total_counts ||
+        h.seed != c->seed || h.eps != c->eps || h.nc != c->nc || bytes < ckpt_bytes(c)) {
+        c->err = "plis_restore: checkpoint of a different model, size, seed, epsilon or n_c";
+        return PLIS_ERR_ARG;
+    }
+    p += sizeof h;
+    for (uint32_t r = 0; r < c->nreal; ++r) {
+        uint32_t N, id;
+        std::memcpy(&N, p + 4ull * r, 4);
+        std::memcpy(&id, p + 4ull * (c->nreal + r), 4);
+        if (N != c->info[r].N || id != c->info[r].id) {
+            c->err = "plis_restore: realization " + std::to_string(r) + " differs in size or id";
+            return PLIS_ERR_ARG;
+        }
+    }
+    p += 8ull * c->nreal;
+    std::vector<RealCtl> ctl(c->nreal);
+    std::memcpy(ctl.data(), p, c->nreal * sizeof(RealCtl));
+    p += c->nreal * sizeof(RealCtl);
+    for (auto& q : ctl) q.cur = 0u;  // counts go to buffer 0
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(c->d_x0, p, c->total_counts * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_ctl, ctl.data(), ctl.size() * sizeof(RealCtl), hipMemcpyHostToDevice));
+    c->ctl_host = ctl;
+    return PLIS_OK;
+}
+
 extern "C" int plis_get_stats(plis_ctx* c, int32_t r, plis_stats* st) {
     if (!c || !st || r < 0 || (uint32_t)r >= c->nreal) return PLIS_ERR_ARG;
     if (int rc = pull_ctl(c)) return rc;

@@ -1,0 +1,68 @@
+"""Checkpoint / resume through the C ABI: a context restored from a
+checkpoint continues bit for bit like the one that wrote it (counts, clock,
+step statistics), for a single grid with leap-backs and critical events and
+for an uneven batch; a checkpoint of another seed or epsilon is refused."""
+import numpy as np
+import pytest
+
+from pliscuda_amd import capi
+from pliscuda_amd import workloads as W
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+STAT_KEYS = ("t", "steps", "leapbacks", "firings", "subvolume_leaps", "crit_events", "status")
+
+
+def _resume_matches(net, topos, xs, eps, seed, first, total, t_end=np.inf):
+    a = capi.Context(net, topos, xs, eps=eps, seed=seed)
+    a.run(t_end, step_limit=first)
+    blob = a.checkpoint()
+    at_ckpt = [(a.stats(r), a.get_state(r)) for r in range(len(topos))]
+    a.run(t_end, step_limit=total)
+    b = capi.Context(net, topos, [np.zeros_like(np.asarray(x)) for x in xs], eps=eps, seed=seed)
+    b.restore(blob)
+    for r, (st, x) in enumerate(at_ckpt):
+        assert np.array_equal(b.get_state(r), x)
+        assert all(b.stats(r)[k] == st[k] for k in STAT_KEYS)
+    b.run(t_end, step_limit=total)
+    for r in range(len(topos)):
+        assert np.array_equal(a.get_state(r), b.get_state(r)), f"realization {r}: state differs"
+        sa, sb = a.stats(r), b.stats(r)
+        for k in STAT_KEYS:
+            assert sa[k] == sb[k], (r, k, sa[k], sb[k])
+    a.close()
+    b.close()
+    return blob
+
+
+def test_resume_low_counts_with_critical_events():
+    net, topo, x0 = H.low_count_case()
+    _resume_matches(net, [topo], [x0], 0.03, 17, 40, 120)
+
+
+def test_resume_brusselator_mesh():
+    net, topo, x0 = H.small_bruss()
+    _resume_matches(net, [topo], [x0], 0.03, 5, 25, 60)
+
+
+def test_resume_uneven_batch_to_t_end():
+    wl = W.abc_reflecting(n=6, realizations=5)
+    _resume_matches(wl["net"], wl["topos"], wl["x0"], 0.05, 3, 30, 10**6, t_end=2.0)
+
+
+def test_restore_refuses_other_seed_or_eps():
+    net, topo, x0 = H.small_bruss()
+    a = capi.Context(net, [topo], [x0], eps=0.03, seed=5)
+    a.run(np.inf, step_limit=3)
+    blob = a.checkpoint()
+    a.close()
+    for kw in ({"eps": 0.03, "seed": 6}, {"eps": 0.05, "seed": 5}):
+        b = capi.Context(net, [topo], [x0], **kw)
+        with pytest.raises(capi.PlisError):
+            b.restore(blob)
+        b.close()
+    b = capi.Context(net, [topo], [x0], eps=0.03, seed=5)
+    with pytest.raises(capi.PlisError):
+        b.restore(blob[:100])
+    b.close()
