@@ -13,6 +13,7 @@
 #include <ctime>
 #include <fstream>
 #include <iostream>
+#include <iterator>
 #include <limits>
 #include <string>
 #include <vector>
@@ -40,6 +41,20 @@ std::string print_state_snapshot(const uint32_t* h_state, int spc, unsigned sbc,
         s += "\n";
     }
     return s;
+}
+
+// The final state in the input State format ("subvolumes: N", "species: S",
+// then "v: c0 c1 ..."), which the reference's snapshots are not.
+static bool write_state_file(const std::string& path, const std::vector<uint32_t>& x, int S, unsigned N) {
+    std::ofstream f(path);
+    if (!f) return false;
+    f << "subvolumes: " << N << "\nspecies: " << S << "\n\n";
+    for (unsigned v = 0; v < N; ++v) {
+        f << v << ":";
+        for (int s = 0; s < S; ++s) f << " " << x[(size_t)s * N + v];
+        f << "\n";
+    }
+    return (bool)f;
 }
 
 static void write_snapshot(std::time_t tstamp, int real, int nreal, const std::vector<uint32_t>& x, int S,
@@ -74,6 +89,18 @@ int run_simulation(Topology t, State s, Reactions r, float* h_rrc, float* h_drc,
         std::cerr << "plis_create failed: " << plis_last_error(nullptr) << "\n";
         return 3;
     }
+    if (!opts.restore.empty()) {
+        std::ifstream in(opts.restore, std::ios::binary);
+        std::vector<char> blob((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+        if (!in.good() && !in.eof()) blob.clear();
+        if (blob.empty() || plis_restore(ctx, blob.data(), blob.size()) != PLIS_OK) {
+            std::cerr << "cannot restore from " << opts.restore << ": "
+                      << (blob.empty() ? "unreadable file" : plis_last_error(ctx)) << "\n";
+            plis_destroy(ctx);
+            return 3;
+        }
+        std::cout << "  [restored from " << opts.restore << "]\n";
+    }
     int32_t shape[3];
     plis_launch_shape(ctx, shape);
     std::cout << "  [using " << shape[1] << " block(s) of " << shape[0] << " wavefront(s), " << shape[2]
@@ -89,6 +116,11 @@ int run_simulation(Topology t, State s, Reactions r, float* h_rrc, float* h_drc,
     const uint64_t no_limit = std::numeric_limits<uint64_t>::max();
     std::vector<uint32_t> x((size_t)S * N);
     double last_log = 0.0;
+    if (!opts.restore.empty()) {
+        plis_stats st;
+        plis_get_stats(ctx, 0, &st);
+        last_log = st.t;  // snapshots continue from the restored clock
+    }
     bool ok = true;
     if (log_freq > 0) {
         for (;;) {
@@ -124,6 +156,13 @@ int run_simulation(Topology t, State s, Reactions r, float* h_rrc, float* h_drc,
         plis_get_stats(ctx, q, &sq);
         plis_get_state(ctx, q, x.data());
         write_snapshot(tstamp, q, nreal, x, S, N, sq.t);
+        if (!opts.state_out.empty()) {
+            const std::string path = nreal > 1 ? opts.state_out + "_r" + std::to_string(q) : opts.state_out;
+            if (!write_state_file(path, x, S, N)) {
+                std::cerr << "cannot write " << path << "\n";
+                ok = false;
+            }
+        }
         steps += sq.steps;
         leapbacks += sq.leapbacks;
         firings += sq.firings;
@@ -138,6 +177,24 @@ int run_simulation(Topology t, State s, Reactions r, float* h_rrc, float* h_drc,
     std::cout << "  firings:                " << firings << "\n";
     std::cout << "  subvolume-leaps/second: " << (el > 0 ? leaps / el : 0.0) << "\n";
     std::cout << "\n";
+    if (!opts.checkpoint.empty()) {
+        uint64_t bytes = 0;
+        std::vector<char> blob;
+        bool wrote = plis_checkpoint_size(ctx, &bytes) == PLIS_OK;
+        if (wrote) {
+            blob.resize(bytes);
+            wrote = plis_checkpoint(ctx, blob.data(), bytes) == PLIS_OK;
+        }
+        if (wrote) {
+            std::ofstream f(opts.checkpoint, std::ios::binary);
+            f.write(blob.data(), (std::streamsize)blob.size());
+            wrote = (bool)f;
+        }
+        if (!wrote) {
+            std::cerr << "cannot write checkpoint " << opts.checkpoint << "\n";
+            ok = false;
+        }
+    }
     plis_destroy(ctx);
     return ok ? 0 : 4;
 }

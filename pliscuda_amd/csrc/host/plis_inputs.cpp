@@ -32,6 +32,12 @@ int header_int(std::istream& is, const std::string& what) {
 
 bool is_blank(const std::string& s) { return s.find_first_not_of(" \t\r") == std::string::npos; }
 
+std::string trim(const std::string& s) {
+    const auto b = s.find_first_not_of(" \t\r");
+    if (b == std::string::npos) return "";
+    return s.substr(b, s.find_last_not_of(" \t\r") - b + 1);
+}
+
 }  // namespace
 
 void Topology::read(std::istream& is) {
@@ -226,6 +232,9 @@ Configuration::Configuration(std::ifstream& cf) {
         else if (key == "nc") opts_.n_critical = std::stoi(val);
         else if (key == "seed") opts_.seed = std::stoull(val);
         else if (key == "realizations") opts_.realizations = std::stoi(val);
+        else if (key == "checkpoint") opts_.checkpoint = trim(val);
+        else if (key == "restore") opts_.restore = trim(val);
+        else if (key == "stateOut") opts_.state_out = trim(val);
         else throw std::invalid_argument("unknown configuration key " + key);
     }
 }

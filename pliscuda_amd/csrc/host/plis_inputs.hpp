@@ -64,6 +64,9 @@ struct Options {
     int n_critical = 10;    // reference value (driver.cu:36)
     uint64_t seed = 0;      // 0: derived from the wall clock
     int realizations = 1;
+    std::string checkpoint;  // write a checkpoint here at the end of the run
+    std::string restore;     // continue from this checkpoint
+    std::string state_out;   // final state in the input State format (re-readable)
 };
 
 class Configuration {

@@ -99,3 +99,49 @@ def test_log_freq_snapshots_do_not_change_the_trajectory(tmp_path):
     lines = open(final).read().split("\n")
     xb = np.array([[int(v) for v in lines[4 + i].split()] for i in range(9)], dtype=np.int64).T
     assert np.array_equal(xa, xb)
+
+
+def test_checkpoint_restore_continues_the_run(tmp_path):
+    """`pliscuda` with checkpoint= stops at endTime 4 and saves the state;
+    a second run with restore= continues to endTime 8 and lands where one
+    context stepping to 4 and then to 8 lands (the run to 4 clamps its last
+    step, so the pair is compared with the same two-leg run through the C
+    ABI).  stateOut= writes the final state as a re-readable State file."""
+    from pliscuda_amd import capi
+    src = os.path.join(CASES, "react-9")
+    a, b = tmp_path / "a", tmp_path / "b"
+    for d in (a, b):
+        d.mkdir()
+        for f in os.listdir(src):
+            if f.endswith(".txt"):
+                shutil.copyfile(os.path.join(src, f), d / f)
+    base = (a / "conf.txt").read_text().rstrip("\n").replace("endTime=8", "endTime=4")
+    (a / "conf.txt").write_text(base + f"\nepsilon=0.05\nnc=10\nseed={SEED}\ncheckpoint=ck.bin\nstateOut=final.txt\n")
+    print("driver react-9 to t=4 with checkpoint", flush=True)
+    r = subprocess.run([DRIVER, "conf.txt"], cwd=a, capture_output=True, text=True, timeout=110)
+    assert r.returncode == 0, r.stdout + r.stderr
+    conf_b = (b / "conf.txt").read_text().rstrip("\n")
+    (b / "conf.txt").write_text(conf_b + f"\nepsilon=0.05\nnc=10\nseed={SEED}\nrestore={a / 'ck.bin'}\n")
+    print("driver react-9 restored to t=8", flush=True)
+    r = subprocess.run([DRIVER, "conf.txt"], cwd=b, capture_output=True, text=True, timeout=110)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "restored from" in r.stdout
+    snaps = glob.glob(str(b / "sim*.dat"))
+    final = max(snaps, key=lambda p: float(re.search(r"_([0-9.]+)\.dat$", p).group(1)))
+    lines = open(final).read().split("\n")
+    N, S = int(lines[0].split("=")[1]), int(lines[1].split("=")[1])
+    assert float(lines[2].split("=")[1]) == pytest.approx(8.0)
+    x = np.array([[int(v) for v in lines[4 + i].split()] for i in range(N)], dtype=np.uint32).T.ravel()
+    c = inputs.read_conf(os.path.join(src, "conf.txt"))
+    ctx = capi.Context(c["net"], [c["topo"]], [c["x0"]], eps=0.05, nc=10, seed=SEED)
+    ctx.run(4.0)
+    x4 = ctx.get_state()
+    ctx.run(8.0)
+    assert np.array_equal(x, ctx.get_state())
+    ctx.close()
+    st = inputs.read_state(str(a / "final.txt"))
+    assert st["N"] == N and st["S"] == S and np.array_equal(st["x"], x4)
+    # a checkpoint of another seed is refused with a non-zero status
+    (b / "conf.txt").write_text(conf_b + f"\nepsilon=0.05\nnc=10\nseed={SEED + 1}\nrestore={a / 'ck.bin'}\n")
+    r = subprocess.run([DRIVER, "conf.txt"], cwd=b, capture_output=True, text=True, timeout=110)
+    assert r.returncode != 0 and "cannot restore" in r.stderr
