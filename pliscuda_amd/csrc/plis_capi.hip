@@ -921,7 +921,7 @@ static int mlp_alloc(plis_ctx* c) {
     std::vector<float> zero(kPTotal + 1, 0.0f), norm(26, 0.0f);
     for (int k = 12; k < 24; ++k) norm[k] = 1.0f;
     norm[25] = 1.0f;
-    c->train_blocks = 256;
+    c->train_blocks = 512;
     if ((rc = upload(c, &c->d_w32, zero)) || (rc = upload(c, &c->d_m, zero)) || (rc = upload(c, &c->d_v, zero)) ||
         (rc = upload(c, &c->d_grad, zero)) || (rc = upload(c, &c->d_norm, norm)) ||
         (rc = dev_alloc(c, &c->d_q, (size_t)kQTotal)) ||
@@ -1034,7 +1034,8 @@ static void train_launch(plis_ctx* c, uint64_t n, uint32_t batch, uint32_t step,
     const uint32_t blocks = std::min<uint32_t>(c->train_blocks, batch / 64);
     hipLaunchKernelGGL(mlp_train_kernel, dim3(blocks), dim3(64), 0, c->stream, ta);
     AdamArgs aa{c->d_w32, c->d_m, c->d_v, c->d_grad, c->d_partial, blocks, adam_t, lr, 0.9f, 0.999f, 1e-8f};
-    hipLaunchKernelGGL(mlp_adam_kernel, dim3((kPTotal + 1 + 255) / 256), dim3(256), 0, c->stream, aa);
+    hipLaunchKernelGGL(mlp_adam_kernel, dim3((kPTotal + 1 + kAdamParams - 1) / kAdamParams), dim3(256), 0,
+                       c->stream, aa);
     if (adam_t > 0)
         hipLaunchKernelGGL(mlp_pack_kernel, dim3((kQTotal + 255) / 256), dim3(256), 0, c->stream, c->d_w32, c->d_q);
 }

@@ -42,7 +42,6 @@ struct MlpDev {
 };
 
 __device__ __forceinline__ float act_f(float z) { return fmaxf(z, 0.0f); }
-__device__ __forceinline__ float act_d(float z) { return z > 0.0f ? 1.0f : 0.0f; }
 
 // lane-layout of v_mfma_f32_32x32x16_bf16: A fragment = 8 consecutive k of
 // row (l % 32) starting at k = 8*(l/32); B fragment = 8 consecutive k of
@@ -68,7 +67,6 @@ struct MlpLds {
     __bf16 h1[64 * kLdH], h1t[64 * kLdH];
     __bf16 h2[64 * kLdH], h2t[64 * kLdH];
     __bf16 dz[64 * kLdH], dzt[64 * kLdH];
-    float g1[64 * 65], g2[64 * 65];  // relu'(z) [row][col]
     float out[64 * 2];
 };
 
@@ -100,10 +98,10 @@ __device__ __forceinline__ void stage_inputs(__bf16* x, __bf16* xt, const MlpDev
 }
 
 // One dense layer on the 64-row tile: out[64][N] = act(in[64][K] . W + b).
-// Writes bf16 activations (row-major and transposed) and relu'(z).
+// Writes bf16 activations (row-major and transposed).
 template <int K, int NB, bool ACT>
 __device__ __forceinline__ void dense(const __bf16* in, int ldin, const __bf16* wt, const float* bias,
-                                      __bf16* h, __bf16* ht, float* gd, float* out2) {
+                                      __bf16* h, __bf16* ht, float* out2) {
 #pragma unroll
     for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
@@ -121,7 +119,6 @@ __device__ __forceinline__ void dense(const __bf16* in, int ldin, const __bf16* 
                     const __bf16 a = (__bf16)act_f(z);
                     h[row * kLdH + col] = a;
                     if (ht) ht[col * kLdH + row] = a;
-                    if (gd) gd[row * 65 + col] = act_d(z);
                 } else if (col < kNo) {
                     out2[row * 2 + col] = c[i] + bias[col];
                 }
@@ -130,17 +127,17 @@ __device__ __forceinline__ void dense(const __bf16* in, int ldin, const __bf16* 
 }
 
 // Forward pass of the tile; returns the lane's two outputs.  Transposed
-// activations and relu'(z) are kept only when h1t/h2t/g1/g2 are given
+// activations are kept only when h1t/h2t are given
 // (training).
 __device__ __forceinline__ void mlp_forward_tile(const __bf16* x, __bf16* h1, __bf16* h1t, __bf16* h2,
-                                                 __bf16* h2t, float* g1, float* g2, float* out,
+                                                 __bf16* h2t, float* out,
                                                  const MlpDev& m, float& o0, float& o1) {
     wave_sync();
-    dense<16, 2, true>(x, kLdX, m.q + kQW1t, m.w32 + kPB1, h1, h1t, g1, nullptr);
+    dense<16, 2, true>(x, kLdX, m.q + kQW1t, m.w32 + kPB1, h1, h1t, nullptr);
     wave_sync();
-    dense<64, 2, true>(h1, kLdH, m.q + kQW2t, m.w32 + kPB2, h2, h2t, g2, nullptr);
+    dense<64, 2, true>(h1, kLdH, m.q + kQW2t, m.w32 + kPB2, h2, h2t, nullptr);
     wave_sync();
-    dense<64, 1, false>(h2, kLdH, m.q + kQW3t, m.w32 + kPB3, nullptr, nullptr, nullptr, out);
+    dense<64, 1, false>(h2, kLdH, m.q + kQW3t, m.w32 + kPB3, nullptr, nullptr, out);
     wave_sync();
     const int r = threadIdx.x & 63;
     o0 = out[r * 2];
@@ -190,7 +187,7 @@ __global__ __launch_bounds__(64) void mlp_train_kernel(TrainArgs a) {
         for (int k = 0; k < 12; ++k) f[k] = a.feats[rec * 12 + k];
         stage_inputs(s.x, s.xt, a.m, f, true);
         float o0, o1;
-        mlp_forward_tile(s.x, s.h1, s.h1t, s.h2, s.h2t, s.g1, s.g2, s.out, a.m, o0, o1);
+        mlp_forward_tile(s.x, s.h1, s.h1t, s.h2, s.h2t, s.out, a.m, o0, o1);
         const float y = a.ylog[rec], c = a.ycrit[rec];
         const float sp = fmaxf(o1, 0.f) + log1pf(expf(-fabsf(o1)));
         loss += (o0 - y) * (o0 - y) + sp - c * o1;
@@ -231,7 +228,8 @@ __global__ __launch_bounds__(64) void mlp_train_kernel(TrainArgs a) {
 #pragma unroll
                 for (int i = 0; i < 16; ++i) {
                     const int row = mb * 32 + acc_row(i), col = nb * 32 + acc_col();
-                    const __bf16 v = (__bf16)(dh[mb * 2 + nb][i] * s.g2[row * 65 + col]);
+                    // relu'(z2) = 1 exactly where the stored activation is positive
+                    const __bf16 v = (__bf16)((float)s.h2[row * kLdH + col] > 0.0f ? dh[mb * 2 + nb][i] : 0.0f);
                     s.dz[row * kLdH + col] = v;
                     s.dzt[col * kLdH + row] = v;
                 }
@@ -265,7 +263,7 @@ __global__ __launch_bounds__(64) void mlp_train_kernel(TrainArgs a) {
 #pragma unroll
                 for (int i = 0; i < 16; ++i) {
                     const int row = mb * 32 + acc_row(i), col = nb * 32 + acc_col();
-                    const __bf16 v = (__bf16)(dh[mb * 2 + nb][i] * s.g1[row * 65 + col]);
+                    const __bf16 v = (__bf16)((float)s.h1[row * kLdH + col] > 0.0f ? dh[mb * 2 + nb][i] : 0.0f);
                     s.dz[row * kLdH + col] = v;
                     s.dzt[col * kLdH + row] = v;
                 }
@@ -329,11 +327,23 @@ struct AdamArgs {
     float lr, b1, b2, eps;
 };
 
+// Block b reduces parameters 64b..64b+63: wave w sums partials w, w+4, ...
+// (coalesced rows), then the four sums are combined in a fixed order, so the
+// gradient is deterministic.
+constexpr int kAdamParams = 64;
 __global__ __launch_bounds__(256) void mlp_adam_kernel(AdamArgs a) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p > kPTotal) return;
+    __shared__ float s_g[4][kAdamParams];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int p = blockIdx.x * kAdamParams + lane;
     float g = 0.f;
-    for (uint32_t q = 0; q < a.nparts; ++q) g += a.partial[(size_t)q * (kPTotal + 1) + p];
+    if (p <= kPTotal) {
+#pragma unroll 4
+        for (uint32_t q = (uint32_t)w; q < a.nparts; q += 4u) g += a.partial[(size_t)q * (kPTotal + 1) + p];
+    }
+    s_g[w][lane] = g;
+    __syncthreads();
+    if (w != 0 || p > kPTotal) return;
+    g = (s_g[0][lane] + s_g[1][lane]) + (s_g[2][lane] + s_g[3][lane]);
     a.grad[p] = g;
     if (p == kPTotal || a.t <= 0) return;
     if (p >= 12 * kNh && p < kNin * kNh) return;  // padding rows of W1 stay zero
