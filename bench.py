@@ -54,6 +54,8 @@ def build_workload(name):
         return W.schlogl()
     if name == "batch":
         return W.schlogl_batch()
+    if name == "bruss_batch":
+        return W.brusselator_batch()
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -93,7 +95,8 @@ def gpu_measure(wl, steps, warmup, seed=2024, autotune=True, shape=None):
             "commit_ms": prof[4] / n_prof, "iter_ms_events": prof[0] / n_prof}
 
 
-def learned_measure(wl, steps, warmup, train_steps, batch, seed=2024, kappa=1.0):
+def learned_measure(wl, steps, warmup, train_steps, batch, seed=2024, kappa=1.0, snapshots=16,
+                    per_snapshot=16384):
     """Records from plain steps of the same workload, GPU training, then the
     learned step timed like the plain one."""
     import torch
@@ -102,10 +105,10 @@ def learned_measure(wl, steps, warmup, train_steps, batch, seed=2024, kappa=1.0)
     N = int(sum(t["N"] for t in wl["topos"]))
     rec = capi.Context(wl["net"], wl["topos"], wl["x0"], eps=wl["eps"], nc=wl["nc"], seed=seed + 1)
     t0 = time.perf_counter()
-    F, Y, Cc = learned.collect_records(rec, snapshots=16, stride=4, per_snapshot=16384)
+    F, Y, Cc = learned.collect_records(rec, snapshots=snapshots, stride=4, per_snapshot=per_snapshot)
     t_rec = time.perf_counter() - t0
     plain_stats = [rec.stats(r) for r in range(rec.n)]
-    vmax = learned.calibrate_viol_rate(plain_stats, N)
+    vmax = learned.calibrate_viol_rate(plain_stats, [int(t["N"]) for t in wl["topos"]])
     t0 = time.perf_counter()
     norm, losses = learned.train(rec, F, Y, Cc, steps=train_steps, batch=batch, lr=1e-3)
     rec.sync()
@@ -173,6 +176,40 @@ def cpu_baseline(wl, seed, sample_steps, gpu_check=True):
             "sample": f"{use} realization(s) x {sample_steps} accepted steps of the same workload "
                       f"from the same initial state and seed ({dt:.1f} s of CPU work)",
             "seconds": dt, "gpu_state_bit_identical": parity}
+
+
+def abc_to_t_end(seed=2024):
+    """The reference's CPU-runnable case (64 realizations of the 8^3 A-B-C
+    reflecting grid to t = 10): GPU wall time to t_end, the oracle on the host
+    cores for the same work, and whether their final states agree bit for bit."""
+    import torch
+
+    from oracle import ffi
+    from pliscuda_amd import capi
+    from pliscuda_amd import workloads as W
+    wl = W.abc_reflecting()
+    ctx = capi.Context(wl["net"], wl["topos"], wl["x0"], eps=wl["eps"], nc=wl["nc"], seed=seed)
+    warm = capi.Context(wl["net"], wl["topos"], wl["x0"], eps=wl["eps"], nc=wl["nc"], seed=seed + 1)
+    warm.run(wl["t_end"])
+    warm.close()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ctx.run(wl["t_end"])
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    st = [ctx.stats(r) for r in range(ctx.n)]
+    leaps = sum(s["subvolume_leaps"] for s in st)
+    gpu_states = [ctx.get_state(r) for r in range(ctx.n)]
+    ctx.close()
+    m = ffi.Model(wl["net"], wl["topos"][0])
+    xs = [np.array(x, dtype=np.uint32).copy() for x in wl["x0"]]
+    t0 = time.perf_counter()
+    ffi.run_batch(m, wl["topos"], xs, wl["eps"], wl["nc"], seed, wl["t_end"])
+    cdt = time.perf_counter() - t0
+    return {"subvolume_leaps_per_s": leaps / dt, "seconds_to_t_end": dt, "steps_per_realization":
+            float(np.mean([s["steps"] for s in st])), "cpu_seconds_to_t_end": cdt,
+            "cpu_subvolume_leaps_per_s": leaps / cdt, "cpu_threads": ffi.num_threads(),
+            "gpu_state_bit_identical": all(np.array_equal(g, x) for g, x in zip(gpu_states, xs))}
 
 
 def copy_probe(nbytes=1 << 28, reps=5):
@@ -292,6 +329,18 @@ def main():
                                  "ms_per_step": r2["seconds"] * 1e3 / 200,
                                  "tau_ms_avg": r2["tau_ms"], "leap_ms_avg": r2["leap_ms"],
                                  "leap_GBs": algorithmic_bytes(1, w2["topos"], "leap") / (r2["leap_ms"] * 1e-3) / 1e9}
+        extra["abc_reflecting_8^3_x64_to_t10"] = abc_to_t_end()
+        wb = build_workload("bruss_batch")
+        rp = gpu_measure(wb, 100, 10)
+        rl = learned_measure(wb, 100, 10, 5000, 65536, snapshots=8, per_snapshot=512)
+        vp, vl = rp["subvolume_leaps"] / rp["seconds"], rl["subvolume_leaps"] / rl["seconds"]
+        sp, sl = rp["subvolume_simtime"] / rp["seconds"], rl["subvolume_simtime"] / rl["seconds"]
+        extra[wb["name"] + "_learned"] = {
+            "plain_subvolume_leaps_per_s": vp, "learned_subvolume_leaps_per_s": vl,
+            "speedup_per_step": vl / vp, "speedup_simulated_time": sl / sp,
+            "plain_ms_per_step": rp["seconds"] * 10.0, "learned_ms_per_step": rl["seconds"] * 10.0,
+            "rejection_rate": rl["rejection_rate"], "leapbacks": rl["leapbacks"], "records": rl["records"],
+            "train_seconds": rl["train_seconds"], "loss_last": rl["loss_last"]}
         out["extra"] = extra
     print(json.dumps(out), flush=True)
 

@@ -188,9 +188,10 @@ int plis_mlp_train(plis_ctx *ctx, const float *feats, const float *ylog, const f
 int plis_records(plis_ctx *ctx, int32_t r, float *feats, float *ylog, float *crit);
 /* learned = 1: tau = kappa * min_v exp(network log tau) (the critical part
  * stays exact); a step is kept only if no population goes negative and at
- * most viol_rate_max * N subvolumes exceed theta * max(eps x / g, 1) in
- * realized change, otherwise it is redone with Cao's tau.  theta also sets
- * the bound the violation statistics use in plain mode. */
+ * most mu + 3 sqrt(mu) + 2 subvolumes, mu = viol_rate_max * N, exceed
+ * theta * max(eps x / g, 1) in realized change (a Poisson upper bound around
+ * the plain method's violation rate), otherwise it is redone with Cao's tau.
+ * theta also sets the bound the violation statistics use in plain mode. */
 int plis_set_tau_mode(plis_ctx *ctx, int32_t learned, double kappa, double theta,
                       double viol_rate_max);
 

@@ -1033,10 +1033,12 @@ __device__ void finalize_B(const KernelArgs& a, const RealInfo& ri, uint32_t r, 
     RealCtl c = a.ctl[r];
     // A learned proposal is judged by Cao's leap condition after the fact: it
     // is kept only without negative populations and with no more violating
-    // subvolumes than the plain method produces (viol_rate_max); otherwise
-    // the step is redone with Cao's tau and fresh draws.
-    const bool learned_reject =
-        c.learned && (bad || (double)viol > a.viol_rate_max * (double)ri.N);
+    // subvolumes than the plain method produces -- mean mu = viol_rate_max N,
+    // with a Poisson upper bound mu + 3 sqrt(mu) + 2 so that small
+    // realizations are not rejected for single violations; otherwise the
+    // step is redone with Cao's tau and fresh draws.
+    const double mu = a.viol_rate_max * (double)ri.N;
+    const bool learned_reject = c.learned && (bad || (double)viol > mu + 3.0 * sqrt(mu) + 2.0);
     if (bad || learned_reject) {
         c.attempt += 1u;
         if (learned_reject) {

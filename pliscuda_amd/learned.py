@@ -81,8 +81,11 @@ def train(ctx, feats, ylog, crit, steps=2000, batch=65536, lr=1e-3, seed=1, w0=N
 
 def calibrate_viol_rate(stats_list, N, margin=2.0):
     """Judge threshold from the plain method's own post-leap violations:
-    margin x the observed mean fraction of violating subvolumes per step."""
+    margin x the observed fraction of violating subvolumes per subvolume-step.
+    N is the realizations' subvolume counts (a list, one per entry of
+    stats_list) or one count shared by all."""
+    Ns = list(N) if np.ndim(N) else [N] * len(stats_list)
     v = sum(s["violations"] for s in stats_list)
-    st = sum(s["steps"] for s in stats_list)
-    rate = v / max(1, st) / N
-    return max(margin * rate, 2.0 / N)
+    sv = sum(s["steps"] * n for s, n in zip(stats_list, Ns))
+    rate = v / max(1, sv)
+    return max(margin * rate, 2.0 / max(Ns))

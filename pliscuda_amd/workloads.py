@@ -232,5 +232,22 @@ def schlogl_batch(realizations=256, lo=16, hi=48, seed=SEED_BATCH):
             "x0": x0, "eps": 0.03, "nc": 10, "t_end": 5.0, "seed": seed, "sides": sides}
 
 
+def brusselator_batch(realizations=128, lo=4, hi=16, seed=SEED_BRUSS + 7):
+    """Uneven batch of Brusselator random meshes (the learned-tau batch): each
+    realization a (2a x 2b x 2c) mesh with a, b, c ~ U{lo..hi}, the headline
+    network and Poisson initial counts."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    dims = 2 * rng.integers(lo, hi + 1, size=(realizations, 3))
+    net = brusselator_network(SEED_BRUSS)
+    means = rng.uniform(20.0, 200.0, size=5)
+    topos, x0 = [], []
+    for r, (nx, ny, nz) in enumerate(dims):
+        t = random_mesh(int(nx), int(ny), int(nz), seed + 100 + r)
+        topos.append(t)
+        x0.append(poisson_state(5, t["N"], means, seed + 1000 + r))
+    return {"name": f"brusselator_batch_{realizations}x[{2 * lo}..{2 * hi}]^3", "net": net, "topos": topos,
+            "x0": x0, "eps": 0.03, "nc": 10, "t_end": float("inf"), "seed": seed}
+
+
 def total_edges(topos):
     return int(sum(len(t["col"]) for t in topos))

@@ -2,6 +2,7 @@
 (double precision, no bfloat16 rounding) and Adam against the published
 update rule."""
 import numpy as np
+import pytest
 
 from oracle import ffi
 from pliscuda_amd import learned
@@ -54,3 +55,15 @@ def test_adam_matches_update_rule():
     ffi.adam_step(w, m, v, g, 1, lr=1e-2)
     # first step: m_hat = g, v_hat = g^2 -> w -= lr * g / (|g| + eps)
     np.testing.assert_allclose(w, w0 - 1e-2 * g / (np.abs(g) + 1e-8), rtol=1e-5, atol=1e-7)
+
+
+def test_calibrate_viol_rate_weights_realizations_by_size():
+    """The plain method's violation rate is per subvolume-step: a batch
+    weights each realization by steps x subvolumes."""
+    from pliscuda_amd import learned
+    stats = [{"violations": 10, "steps": 100}, {"violations": 0, "steps": 50}]
+    r = learned.calibrate_viol_rate(stats, [1000, 10], margin=1.0)
+    assert r == pytest.approx(10 / (100 * 1000 + 50 * 10))
+    one = learned.calibrate_viol_rate([{"violations": 10, "steps": 100}], 1000, margin=2.0)
+    assert one == pytest.approx(2 * 10 / (100 * 1000))
+    assert learned.calibrate_viol_rate([{"violations": 0, "steps": 5}], 100) == pytest.approx(2.0 / 100)
