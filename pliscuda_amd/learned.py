@@ -87,5 +87,4 @@ def calibrate_viol_rate(stats_list, N, margin=2.0):
     Ns = list(N) if np.ndim(N) else [N] * len(stats_list)
     v = sum(s["violations"] for s in stats_list)
     sv = sum(s["steps"] * n for s, n in zip(stats_list, Ns))
-    rate = v / max(1, sv)
-    return max(margin * rate, 2.0 / max(Ns))
+    return margin * v / max(1, sv)

@@ -66,4 +66,5 @@ def test_calibrate_viol_rate_weights_realizations_by_size():
     assert r == pytest.approx(10 / (100 * 1000 + 50 * 10))
     one = learned.calibrate_viol_rate([{"violations": 10, "steps": 100}], 1000, margin=2.0)
     assert one == pytest.approx(2 * 10 / (100 * 1000))
-    assert learned.calibrate_viol_rate([{"violations": 0, "steps": 5}], 100) == pytest.approx(2.0 / 100)
+    # no observed violations: rate 0, the judge's Poisson bound still allows 2 per step
+    assert learned.calibrate_viol_rate([{"violations": 0, "steps": 5}], 100) == 0.0
