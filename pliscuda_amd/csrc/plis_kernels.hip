@@ -93,6 +93,16 @@ __device__ __forceinline__ double binom_d(uint32_t x, int r) {
     return ((xd * (xd - 1.0)) * (xd - 2.0)) / 6.0;
 }
 
+// floor(x / r) for a reactant coefficient r (1..3 for orders up to 3, any r
+// handled): shifts and a division by the constant 3 instead of a runtime
+// 32-bit division.
+__device__ __forceinline__ uint32_t div_coef(uint32_t x, int r) {
+    if (r == 1) return x;
+    if (r == 2) return x >> 1;
+    if (r == 3) return x / 3u;
+    return x / (uint32_t)r;
+}
+
 __device__ __forceinline__ bool hop_active(double d, uint32_t x, int nc) {
     return d > 0.0 && (long long)x >= (long long)nc && x > 0u;
 }
@@ -143,7 +153,7 @@ __device__ __forceinline__ void react_props(const NetView& n, const uint32_t (&x
             for (int q = 0; q < MS; ++q)
                 if (q == s) xs = x[q];
             aj = aj * binom_d(xs, r);
-            const long long l = (long long)(xs / (uint32_t)r);
+            const long long l = (long long)div_coef(xs, r);
             if (L < 0 || l < L) L = l;
         }
         av[j] = aj;
@@ -863,7 +873,7 @@ __device__ __forceinline__ void walk_tile_draws(const KernelArgs& a, const NetVi
             const int sq = n.rs[j * 3 + q], rq = n.rr[j * 3 + q];
             const uint32_t xs = s_x[sq][lane];
             aj = aj * binom_d(xs, rq);
-            const long long l = (long long)(xs / (uint32_t)rq);
+            const long long l = (long long)div_coef(xs, rq);
             if (L < 0 || l < L) L = l;
         }
         if ((n.order[j] > 0 && L < (long long)nc) || !(aj > 0.0)) continue;

@@ -198,3 +198,34 @@ def test_paused_run_equals_straight_run():
     assert a.stats()["steps"] == b.stats()["steps"]
     a.close()
     b.close()
+
+
+def _violations_match(net, topo, x0, eps, steps=15, seed=9):
+    """Every accepted step adds the subvolumes whose realized change broke
+    theta * max(eps x / g, 1) (theta = 3) to the statistics; step by step this
+    must be the oracle's count between the same two states."""
+    ctx = capi.Context(net, [topo], [x0], eps=eps, seed=seed)
+    m = ffi.Model(net, topo)
+    prev, before, total = np.array(x0, dtype=np.uint32), 0, 0
+    for k in range(1, steps + 1):
+        ctx.run(np.inf, step_limit=k)
+        x = ctx.get_state()
+        v = ctx.stats()["violations"]
+        n, _ = ffi.violations(m, prev, x, eps, 10, 3.0)
+        assert v - before == n, (k, v - before, n)
+        total += n
+        prev, before = x, v
+    ctx.close()
+    return total
+
+
+@pytest.mark.parametrize("eps", [0.03, 0.1])
+def test_violation_counts_match_oracle(eps):
+    net, topo, x0 = H.small_bruss()
+    assert _violations_match(net, topo, x0, eps) > 0
+
+
+def test_violation_counts_match_oracle_asymmetric():
+    net, topo, x0 = H.small_bruss()
+    atopo = H.asym_topology(topo["N"], 3)
+    _violations_match(net, atopo, x0, 0.05)
