@@ -43,6 +43,9 @@ struct plis_ctx {
     uint32_t *d_x0 = nullptr, *d_x1 = nullptr;
     uint32_t *d_rp = nullptr, *d_col = nullptr, *d_irp = nullptr, *d_icol = nullptr;
     double* d_wsum = nullptr;  // per subvolume, row_ptr layout
+    uint32_t *d_frp = nullptr, *d_fcol = nullptr;  // forward CSR (symmetric realizations)
+    float* d_fw = nullptr;
+    unsigned long long* d_gdelta = nullptr;        // leap balances, counts layout
     float *d_w = nullptr, *d_iw = nullptr;
     int32_t* d_comp = nullptr;
     PartA* d_pa_t = nullptr;
@@ -55,7 +58,7 @@ struct plis_ctx {
     uint64_t seed = 0;
     uint32_t wpb = 4, blocks = 8;
     int use_graph = 1;
-    bool edge_parallel = false;  // every realization symmetric: edge-parallel leap tiles
+    bool edge_parallel = false;  // every realization symmetric, tiles < 2^27 forward entries
     hipGraphExec_t graph = nullptr;
     uint32_t graph_iters = 0;
     std::vector<void*> allocs;
@@ -267,6 +270,10 @@ static KernelArgs make_args(plis_ctx* c) {
     a.ndone = c->d_cnt;
     a.redo_count = c->d_cnt + 1;
     a.redo_tiles = c->d_redo;
+    a.frp = c->d_frp;
+    a.fcol = c->d_fcol;
+    a.fw = c->d_fw;
+    a.gdelta = c->d_gdelta;
     a.redo_total = reinterpret_cast<unsigned long long*>(c->d_cnt + 2);
     a.epsilon = c->eps;
     a.n_critical = c->nc;
@@ -309,9 +316,11 @@ static void launch_variant(plis_ctx* c, int which, const KernelArgs& a) {
                     if (mc) {
                         hipLaunchKernelGGL((phase_b_kernel<MS, MR, true, true>), grid, block, 0, c->stream, a);
                         hipLaunchKernelGGL((phase_b_redo_kernel<MS, MR, true>), grid, block, 0, c->stream, a);
+                        hipLaunchKernelGGL((phase_b_finish_kernel<MS, MR, true>), grid, block, 0, c->stream, a);
                     } else {
                         hipLaunchKernelGGL((phase_b_kernel<MS, MR, true, false>), grid, block, 0, c->stream, a);
                         hipLaunchKernelGGL((phase_b_redo_kernel<MS, MR, false>), grid, block, 0, c->stream, a);
+                        hipLaunchKernelGGL((phase_b_finish_kernel<MS, MR, false>), grid, block, 0, c->stream, a);
                     }
                     break;
                 }
@@ -449,12 +458,13 @@ extern "C" int plis_create(const plis_network* net, int32_t n_real, const plis_t
     c->nreal = (uint32_t)n_real;
 
     // host-side layout
-    std::vector<uint32_t> hx, hrp, hcol, hirp, hicol;
+    std::vector<uint32_t> hx, hrp, hcol, hirp, hicol, hfrp, hfcol;
+    std::vector<float> hfw;
     std::vector<float> hw, hiw;
     std::vector<double> hwsum;
     std::vector<int32_t> hcomp;
     std::vector<TileEnt> tiles;
-    bool any_weighted = false, any_asym = false;
+    bool any_weighted = false, any_asym = false, huge_tile = false;
     uint64_t xoff = 0, eoff = 0, rpoff = 0, ieoff = 0, irpoff = 0, coff = 0;
     uint32_t tile = 0;
     for (int32_t r = 0; r < n_real; ++r) {
@@ -481,6 +491,19 @@ extern "C" int plis_create(const plis_network* net, int32_t n_real, const plis_t
         any_weighted = any_weighted || weighted;
         hrp.insert(hrp.end(), g.rp.begin(), g.rp.end());
         hcol.insert(hcol.end(), g.col.begin(), g.col.end());
+        // forward CSR: each undirected edge once, in the row of its lower end
+        ri.fe_off = hfcol.size();
+        for (uint32_t v = 0; v < N; ++v) {
+            hfrp.push_back((uint32_t)(hfcol.size() - ri.fe_off));
+            for (uint32_t q = g.rp[v]; q < g.rp[v + 1]; ++q)
+                if (g.col[q] > v) {
+                    hfcol.push_back(g.col[q]);
+                    hfw.push_back(g.w[q]);
+                }
+        }
+        hfrp.push_back((uint32_t)(hfcol.size() - ri.fe_off));
+        for (uint32_t v = 0; v < N; v += kTile)  // entry offsets of a tile's queue codes
+            if (hfrp[rpoff + std::min(v + (uint32_t)kTile, N)] - hfrp[rpoff + v] >= kMaxTileEdges) huge_tile = true;
         hw.insert(hw.end(), g.w.begin(), g.w.end());
         // W_v = sum of out-edge weights, summed in CSR order in double
         for (uint32_t v = 0; v < N; ++v) {
@@ -563,7 +586,16 @@ extern "C" int plis_create(const plis_network* net, int32_t n_real, const plis_t
         plis_destroy(c);
         return rc;
     }
-    c->edge_parallel = !any_asym;
+    // the forward CSR and the zeroed balances serve the edge-parallel leap
+    const bool ep = !any_asym && !huge_tile;
+    if (ep && ((rc = upload(c, &c->d_frp, hfrp)) || (rc = upload(c, &c->d_fcol, hfcol)) ||
+                      (rc = upload(c, &c->d_fw, any_weighted ? hfw : std::vector<float>())) ||
+                      (rc = upload(c, &c->d_gdelta, std::vector<unsigned long long>(hx.size(), 0ull))))) {
+        g_create_err = c->err;
+        plis_destroy(c);
+        return rc;
+    }
+    c->edge_parallel = ep;
     if (hipHostMalloc((void**)&c->h_ndone, sizeof(uint32_t)) != hipSuccess) {
         g_create_err = "hipHostMalloc failed";
         plis_destroy(c);

@@ -695,29 +695,35 @@ __device__ __forceinline__ bool leap_subvolume(const KernelArgs& a, const NetVie
 // ---------------------------------------------------------------------------
 // Edge-parallel leap of one tile (symmetric realizations).
 //
-// The tile's CSR edges are contiguous; lane l takes edges l, l+64, ...
-// (coalesced neighbour-id and weight loads, no divergence from uneven
-// degrees), finds the owning subvolume by binary search over the tile's row
-// pointers staged in LDS, and draws both hop directions of every species
-// from the edge's Philox blocks; then lane i draws the reactions of
-// subvolume v0+i.  Every balance goes into LDS with integer atomics (sums
-// are order-free, so this is exact).
+// Each undirected edge {v, u}, v < u, is drawn once, by the tile of its
+// lower end: the tile's forward-CSR entries are contiguous, lane l takes
+// entries l, l+64, ... (coalesced loads, no divergence from uneven degrees),
+// finds the owning subvolume by binary search over the tile's row pointers
+// staged in LDS, and draws both hop directions of every species from the
+// edge's Philox block; then lane i draws the reactions of subvolume v0+i.
+// Balances of the tile's own subvolumes collect in LDS, those of other
+// tiles' subvolumes go straight to the global balances (gdelta); integer
+// atomics keep every sum exact and order-free.  phase_b_finish_kernel adds
+// the balances to the counts once every draw of the pass is in.
 //
-// Two tiers of draws.  phase_b_kernel's walk runs only poisson_fast (zero
-// shortcut and float squeeze); a draw it cannot settle is appended to the
-// wavefront's LDS queue as a 32-bit code and drawn after the walk with the
-// double search, inline, where almost nothing else is live.  A tile whose
-// queue overflows, or that holds a draw of mean >= 10 (PTRS), writes
-// nothing: it goes on the redo list and phase_b_redo_kernel draws it again
-// from scratch with the complete sampler (FULL).  Draws are functions of
-// their counters, so the redo gives the tile exactly the values a single
-// pass would.  No call and no PTRS in phase_b_kernel keeps its register
-// footprint, and so its occupancy, at that of the fast path.
+// Two tiers of draws.  phase_b_kernel's walk runs only the fast samplers
+// (zero shortcut and float squeeze) and applies what they settle; a draw
+// they leave open goes to the wavefront's LDS queue as a 32-bit code and is
+// drawn after the walk with the double search, inline, where almost nothing
+// else is live.  A tile whose queue overflows, or that holds a draw of mean
+// >= 10 (PTRS), applies its settled draws only and goes on the redo list:
+// phase_b_redo_kernel walks it again and draws exactly the draws the fast
+// samplers leave open (the same test on the same inputs) with the complete
+// sampler.  Draws are functions of their counters, so the two passes give
+// the tile the values one pass would.  No call and no PTRS in phase_b_kernel
+// keeps its register footprint, and so its occupancy, at that of the fast
+// path.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kQueue = 128;               // codes per wavefront
 constexpr uint32_t kQueueReact = 0x80000000u;  // reaction code: flag | lane << 8 | j
 constexpr uint32_t kQueueDefer = kQueue + 1u;  // count bump of a draw the queue does not take
 constexpr float kPtrsSureF = 10.0001f;         // a float mean above it is a double mean >= 10
+constexpr uint32_t kMaxTileEdges = 1u << 27;   // hop codes hold the entry offset in 27 bits
 
 // A mean certainly >= 10 (PTRS) sends the tile to the redo pass; others
 // are queued (the drain re-forms the mean in double and may still defer).
@@ -730,7 +736,7 @@ __device__ __forceinline__ void q_append(uint32_t* s_q, uint32_t* s_qn, bool ptr
     if (slot < kQueue) s_q[slot] = code;
 }
 
-// Owner of edge e of the tile: last i with s_rp[i] <= e.
+// Owner of entry e of the tile: last i with s_rp[i] <= e.
 __device__ __forceinline__ uint32_t edge_owner(const uint32_t* s_rp, uint32_t e) {
     uint32_t lo_i = 0, hi_i = kTile - 1;
 #pragma unroll
@@ -752,52 +758,46 @@ __device__ __forceinline__ float hop_mean_f(double d, uint32_t x, float w, float
     return hop_active(d, x, nc) ? tauf * (((float)d * (float)x) * w) : 0.0f;
 }
 
-// One draw of the walk: complete (FULL) or fast with the queue; returns the
-// count to apply now (0 for a queued draw).
-template <bool FULL>
-__device__ __forceinline__ long long walk_draw(double lam, const U4& o, const DrawId& id, uint32_t* s_q,
-                                               uint32_t* s_qn, uint32_t code) {
-    if constexpr (FULL) {
-        return poisson(lam, pick(o, id.word), id);
-    } else {
-        const int k = poisson_fast(lam, pick(o, id.word));
-        if (k >= 0) return k;
-        q_append(s_q, s_qn, !(lam < kPoissonInvMax), code);
-        return 0;
-    }
+// Balance b of species s at subvolume u: LDS when u is in the tile.
+__device__ __forceinline__ void add_balance(const KernelArgs& a, const RealInfo& ri,
+                                            unsigned long long (*s_del)[kTile], uint32_t v0, int s, uint32_t u,
+                                            long long b) {
+    if (u - v0 < (uint32_t)kTile) atomicAdd(&s_del[s][u - v0], (unsigned long long)b);
+    else atomicAdd(&a.gdelta[ri.x_off + (uint64_t)s * ri.N + u], (unsigned long long)b);
 }
 
-// The lane's draws: its edges, then the reactions of its subvolume.
-template <int MS, int MR, bool FULL>
-__device__ __forceinline__ void walk_tile_draws(const KernelArgs& a, const NetView& n, const RealInfo& ri,
-                                                const uint32_t* X, uint32_t v0, const StepView& sv,
-                                                uint32_t (*s_x)[kTile], unsigned long long (*s_del)[kTile],
-                                                const uint32_t* s_rp, const int32_t* s_c, uint32_t* s_q,
-                                                uint32_t* s_qn, unsigned long long& fire) {
+// The lane's draws: its forward entries (both directions of every species
+// of an undirected edge), then the reactions of its subvolume.  Fast walk:
+// settled draws applied, open ones queued.  REDO: the open draws only, with
+// the complete sampler.
+template <int MS, int MR, bool REDO>
+__device__ __forceinline__ void walk_fwd_draws(const KernelArgs& a, const NetView& n, const RealInfo& ri,
+                                               const uint32_t* X, uint32_t v0, const StepView& sv,
+                                               const uint32_t (*s_x)[kTile], unsigned long long (*s_del)[kTile],
+                                               const uint32_t* s_rp, const int32_t* s_c, uint32_t* s_q,
+                                               uint32_t* s_qn, unsigned long long& fire) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t N = ri.N;
     const int nc = a.n_critical;
     const bool weighted = (ri.flags & 2u) != 0;
-    const uint32_t* colp = a.col + ri.e_off;
-    const float* wp = a.w + ri.e_off;
+    const uint32_t* colp = a.fcol + ri.fe_off;
+    const float* wp = a.fw + ri.fe_off;
     const double tau = sv.tau;
     const float tauf = (float)tau;
     const uint32_t k0 = a.seed_lo, k1 = a.seed_hi;
     const uint32_t c3hop = ctr_word3(sv.real, sv.attempt, kHop, 0u);
     const uint32_t eb = s_rp[0], ee = s_rp[kTile];
-    // fast-path hop firings of this lane: each settled draw is <= 15 and a
-    // tile has < 2^27 edges, so 32 bits hold them
+    // fast-path firings of this lane: each settled draw is <= 15 and a tile
+    // has < 2^27 entries, so 32 bits hold them
     uint32_t fire_fast = 0u;
     for (uint32_t e = eb + lane; e < ee; e += (uint32_t)kTile) {
         const uint32_t i = edge_owner(s_rp, e);
         const uint32_t vo = v0 + i;
-        const uint32_t u = colp[e];
+        const uint32_t u = colp[e];  // u > vo: vo is the edge's lower end
         const float wf = weighted ? wp[e] : 1.0f;
-        const double wq = (double)wf;
+        const bool local = u - v0 < (uint32_t)kTile;
         const int co = n.C > 1 ? s_c[i] : 0;
-        const int cu = n.C > 1 ? a.comp[ri.comp_off + u] : 0;
-        const uint32_t lo = vo < u ? vo : u, hi = vo < u ? u : vo;
-        const uint32_t dout = vo > u ? 1u : 0u, din = 1u - dout;
+        const int cu = n.C > 1 ? (local ? s_c[u - v0] : a.comp[ri.comp_off + u]) : 0;
         const uint32_t ecode = (e - eb) << 4;
 #pragma unroll
         for (int s0 = 0; s0 < MS; s0 += 2) {
@@ -806,54 +806,51 @@ __device__ __forceinline__ void walk_tile_draws(const KernelArgs& a, const NetVi
             const int s1 = s0 + 1 < MS ? s0 + 1 : s0;
             const uint32_t xo0 = s_x[s0][i], xo1 = two ? s_x[s1][i] : 0u;
             const double do0 = n.d[co * n.S + s0], do1 = two ? n.d[co * n.S + s1] : 0.0;
-            const uint32_t xu0 = X[ri.x_off + (uint64_t)s0 * N + u];
-            const uint32_t xu1 = two ? X[ri.x_off + (uint64_t)s1 * N + u] : 0u;
+            const uint32_t xu0 = local ? s_x[s0][u - v0] : X[ri.x_off + (uint64_t)s0 * N + u];
+            const uint32_t xu1 = two ? (local ? s_x[s1][u - v0] : X[ri.x_off + (uint64_t)s1 * N + u]) : 0u;
             const double du0 = n.d[cu * n.S + s0], du1 = two ? n.d[cu * n.S + s1] : 0.0;
             const uint32_t blk = (uint32_t)s0 >> 1;
             const uint32_t cb = c3hop | blk;
-            const uint32_t wcode = ecode | (blk << 2);
-            std::conditional_t<FULL, long long, int> b0 = 0, b1 = 0;
-            if constexpr (FULL) {
-                const double lo0 = hop_mean(do0, xo0, wq, tau, nc);
-                const double lo1 = two ? hop_mean(do1, xo1, wq, tau, nc) : 0.0;
-                const double li0 = hop_mean(du0, xu0, wq, tau, nc);
-                const double li1 = two ? hop_mean(du1, xu1, wq, tau, nc) : 0.0;
-                if (!(lo0 > 0.0 || lo1 > 0.0 || li0 > 0.0 || li1 > 0.0)) continue;
-                const U4 o = philox10(lo, hi, sv.step, cb, k0, k1);
-                if (lo0 > 0.0) {
-                    const long long k = poisson(lo0, pick(o, dout), DrawId{lo, hi, sv.step, cb, k0, k1, dout});
-                    b0 -= k;
-                    fire += (unsigned long long)k;
-                }
-                if (li0 > 0.0) b0 += poisson(li0, pick(o, din), DrawId{lo, hi, sv.step, cb, k0, k1, din});
-                if (lo1 > 0.0) {
-                    const long long k =
-                        poisson(lo1, pick(o, 2u + dout), DrawId{lo, hi, sv.step, cb, k0, k1, 2u + dout});
-                    b1 -= k;
-                    fire += (unsigned long long)k;
-                }
-                if (li1 > 0.0) b1 += poisson(li1, pick(o, 2u + din), DrawId{lo, hi, sv.step, cb, k0, k1, 2u + din});
+            // words: 2 (s & 1) + 0 is the hop vo -> u (vo < u), + 1 the hop u -> vo
+            const float lf0 = hop_mean_f(do0, xo0, wf, tauf, nc), lr0 = hop_mean_f(du0, xu0, wf, tauf, nc);
+            const float lf1 = two ? hop_mean_f(do1, xo1, wf, tauf, nc) : 0.0f;
+            const float lr1 = two ? hop_mean_f(du1, xu1, wf, tauf, nc) : 0.0f;
+            if (!(lf0 > 0.0f || lf1 > 0.0f || lr0 > 0.0f || lr1 > 0.0f)) continue;
+            const U4 o = philox10(vo, u, sv.step, cb, k0, k1);
+            std::conditional_t<REDO, long long, int> f0 = 0, r0 = 0, f1 = 0, r1 = 0;  // forward / reverse hops
+            if constexpr (REDO) {
+                const double wq = (double)wf;
+                if (poisson_fast_f(lf0, o.x) < 0)
+                    f0 = poisson(hop_mean(do0, xo0, wq, tau, nc), o.x, DrawId{vo, u, sv.step, cb, k0, k1, 0u});
+                if (poisson_fast_f(lr0, o.y) < 0)
+                    r0 = poisson(hop_mean(du0, xu0, wq, tau, nc), o.y, DrawId{vo, u, sv.step, cb, k0, k1, 1u});
+                if (poisson_fast_f(lf1, o.z) < 0)
+                    f1 = poisson(hop_mean(do1, xo1, wq, tau, nc), o.z, DrawId{vo, u, sv.step, cb, k0, k1, 2u});
+                if (poisson_fast_f(lr1, o.w) < 0)
+                    r1 = poisson(hop_mean(du1, xu1, wq, tau, nc), o.w, DrawId{vo, u, sv.step, cb, k0, k1, 3u});
+                fire += (unsigned long long)(f0 + r0 + f1 + r1);
             } else {
-                const float lo0 = hop_mean_f(do0, xo0, wf, tauf, nc);
-                const float lo1 = two ? hop_mean_f(do1, xo1, wf, tauf, nc) : 0.0f;
-                const float li0 = hop_mean_f(du0, xu0, wf, tauf, nc);
-                const float li1 = two ? hop_mean_f(du1, xu1, wf, tauf, nc) : 0.0f;
-                if (!(lo0 > 0.0f || lo1 > 0.0f || li0 > 0.0f || li1 > 0.0f)) continue;
-                const U4 o = philox10(lo, hi, sv.step, cb, k0, k1);
+                const uint32_t wcode = ecode | (blk << 2);
                 int k;
-                if ((k = poisson_fast_f(lo0, pick(o, dout))) < 0) q_append(s_q, s_qn, lo0 > kPtrsSureF, wcode | dout);
-                else { b0 -= k; fire_fast += (uint32_t)k; }
-                if ((k = poisson_fast_f(li0, pick(o, din))) < 0) q_append(s_q, s_qn, li0 > kPtrsSureF, wcode | din);
-                else b0 += k;
-                if ((k = poisson_fast_f(lo1, pick(o, 2u + dout))) < 0)
-                    q_append(s_q, s_qn, lo1 > kPtrsSureF, wcode | (2u + dout));
-                else { b1 -= k; fire_fast += (uint32_t)k; }
-                if ((k = poisson_fast_f(li1, pick(o, 2u + din))) < 0)
-                    q_append(s_q, s_qn, li1 > kPtrsSureF, wcode | (2u + din));
-                else b1 += k;
+                if ((k = poisson_fast_f(lf0, o.x)) < 0) q_append(s_q, s_qn, lf0 > kPtrsSureF, wcode | 0u);
+                else f0 = k;
+                if ((k = poisson_fast_f(lr0, o.y)) < 0) q_append(s_q, s_qn, lr0 > kPtrsSureF, wcode | 1u);
+                else r0 = k;
+                if ((k = poisson_fast_f(lf1, o.z)) < 0) q_append(s_q, s_qn, lf1 > kPtrsSureF, wcode | 2u);
+                else f1 = k;
+                if ((k = poisson_fast_f(lr1, o.w)) < 0) q_append(s_q, s_qn, lr1 > kPtrsSureF, wcode | 3u);
+                else r1 = k;
+                fire_fast += (uint32_t)(f0 + r0 + f1 + r1);
             }
-            if (b0 != 0) atomicAdd(&s_del[s0][i], (unsigned long long)(long long)b0);
-            if (b1 != 0) atomicAdd(&s_del[s1][i], (unsigned long long)(long long)b1);
+            const long long b0 = (long long)r0 - (long long)f0, b1 = (long long)r1 - (long long)f1;
+            if (b0 != 0) {
+                atomicAdd(&s_del[s0][i], (unsigned long long)b0);
+                add_balance(a, ri, s_del, v0, s0, u, -b0);
+            }
+            if (b1 != 0) {
+                atomicAdd(&s_del[s1][i], (unsigned long long)b1);
+                add_balance(a, ri, s_del, v0, s1, u, -b1);
+            }
         }
     }
     fire += fire_fast;
@@ -867,24 +864,32 @@ __device__ __forceinline__ void walk_tile_draws(const KernelArgs& a, const NetVi
     const uint32_t c3base = ctr_word3(sv.real, sv.attempt, kReact, 0u);
     for (int j = 0; j < n.R; ++j) {
         double aj = n.k[c * n.R + j];
-        long long L = -1;
+        long long L_j = -1;
         const int nr = n.nrs[j];
         for (int q = 0; q < nr; ++q) {
             const int sq = n.rs[j * 3 + q], rq = n.rr[j * 3 + q];
             const uint32_t xs = s_x[sq][lane];
             aj = aj * binom_d(xs, rq);
             const long long l = (long long)div_coef(xs, rq);
-            if (L < 0 || l < L) L = l;
+            if (L_j < 0 || l < L_j) L_j = l;
         }
-        if ((n.order[j] > 0 && L < (long long)nc) || !(aj > 0.0)) continue;
+        if ((n.order[j] > 0 && L_j < (long long)nc) || !(aj > 0.0)) continue;
         const uint32_t b = (uint32_t)j >> 2;
         if (b != blk) {
             o = philox10(v, kSelfId, sv.step, c3base | b, k0, k1);
             blk = b;
         }
         const uint32_t wd = (uint32_t)j & 3u;
-        const long long k = walk_draw<FULL>(tau * aj, o, DrawId{v, kSelfId, sv.step, c3base | b, k0, k1, wd}, s_q,
-                                            s_qn, kQueueReact | (lane << 8) | (uint32_t)j);
+        const double lam = tau * aj;
+        long long k;
+        const int kf = poisson_fast(lam, pick(o, wd));
+        if constexpr (REDO) {
+            if (kf >= 0) continue;
+            k = poisson(lam, pick(o, wd), DrawId{v, kSelfId, sv.step, c3base | b, k0, k1, wd});
+        } else {
+            if (kf < 0) q_append(s_q, s_qn, !(lam < kPoissonInvMax), kQueueReact | (lane << 8) | (uint32_t)j);
+            k = kf < 0 ? 0 : kf;
+        }
         if (k == 0) continue;
         fire += (unsigned long long)k;
         for (int s = 0; s < n.S; ++s) {
@@ -894,15 +899,48 @@ __device__ __forceinline__ void walk_tile_draws(const KernelArgs& a, const NetVi
     }
 }
 
-// One queued draw, re-derived from its code with its mean in double; the
-// double search.  Returns true (nothing applied) for a mean >= 10, which
-// only the redo pass draws.
+// A queued hop, decoded: owner i, neighbour u, species s, word wd and the
+// mean in double.
+struct QueuedHop {
+    uint32_t i, u, blk, wd;
+    int s;
+    bool fwd;  // the hop v0 + i -> u
+    double lam;
+};
+
 template <int MS>
-__device__ __forceinline__ bool drain_draw(const KernelArgs& a, const NetView& n, const RealInfo& ri,
-                                           const uint32_t* X, uint32_t v0, const StepView& sv,
-                                           uint32_t (*s_x)[kTile], unsigned long long (*s_del)[kTile],
-                                           const uint32_t* s_rp, const int32_t* s_c, uint32_t code,
-                                           unsigned long long& fire) {
+__device__ __forceinline__ QueuedHop queued_hop(const KernelArgs& a, const NetView& n, const RealInfo& ri,
+                                                const uint32_t* X, uint32_t v0, const StepView& sv,
+                                                const uint32_t (*s_x)[kTile], const uint32_t* s_rp,
+                                                const int32_t* s_c, uint32_t code) {
+    QueuedHop h;
+    const uint32_t e = s_rp[0] + (code >> 4), w = code & 15u;
+    h.blk = w >> 2;
+    h.wd = w & 3u;
+    h.i = edge_owner(s_rp, e);
+    h.u = a.fcol[ri.fe_off + e];
+    h.s = 2 * (int)h.blk + (int)(h.wd >> 1);
+    h.fwd = (h.wd & 1u) == 0u;
+    const double wq = (ri.flags & 2u) ? (double)a.fw[ri.fe_off + e] : 1.0;
+    if (h.fwd) {
+        h.lam = hop_mean(n.d[(n.C > 1 ? s_c[h.i] : 0) * n.S + h.s], s_x[h.s][h.i], wq, sv.tau, a.n_critical);
+    } else {
+        const bool local = h.u - v0 < (uint32_t)kTile;
+        const int cu = n.C > 1 ? (local ? s_c[h.u - v0] : a.comp[ri.comp_off + h.u]) : 0;
+        const uint32_t xu = local ? s_x[h.s][h.u - v0] : X[ri.x_off + (uint64_t)h.s * ri.N + h.u];
+        h.lam = hop_mean(n.d[cu * n.S + h.s], xu, wq, sv.tau, a.n_critical);
+    }
+    return h;
+}
+
+// One queued draw with its mean in double: the double search (the drain
+// runs only when every queued mean is below 10).
+template <int MS>
+__device__ __forceinline__ void drain_fwd_draw(const KernelArgs& a, const NetView& n, const RealInfo& ri,
+                                               const uint32_t* X, uint32_t v0, const StepView& sv,
+                                               const uint32_t (*s_x)[kTile], unsigned long long (*s_del)[kTile],
+                                               const uint32_t* s_rp, const int32_t* s_c, uint32_t code,
+                                               unsigned long long& fire) {
     const uint32_t k0 = a.seed_lo, k1 = a.seed_hi;
     if (code & kQueueReact) {
         const uint32_t i = (code >> 8) & 63u, j = code & 255u;
@@ -911,53 +949,33 @@ __device__ __forceinline__ bool drain_draw(const KernelArgs& a, const NetView& n
         for (int q = 0; q < nr; ++q) aj = aj * binom_d(s_x[n.rs[j * 3 + q]][i], n.rr[j * 3 + q]);
         const U4 o = philox10(v0 + i, kSelfId, sv.step, ctr_word3(sv.real, sv.attempt, kReact, 0u) | (j >> 2), k0, k1);
         const long long k = inversion_search(sv.tau * aj, pick(o, j & 3u));
-        if (k == 0) return false;
+        if (k == 0) return;
         fire += (unsigned long long)k;
         for (int s = 0; s < n.S; ++s) {
             const int nu = n.nu[s * n.R + (int)j];
             if (nu != 0) atomicAdd(&s_del[s][i], (unsigned long long)(k * (long long)nu));
         }
-        return false;
+        return;
     }
-    const uint32_t e = s_rp[0] + (code >> 4), w = code & 15u;
-    const uint32_t blk = w >> 2, wd = w & 3u;
-    const uint32_t i = edge_owner(s_rp, e);
-    const uint32_t vo = v0 + i;
-    const uint32_t u = a.col[ri.e_off + e];
-    const double wq = (ri.flags & 2u) ? (double)a.w[ri.e_off + e] : 1.0;
-    const uint32_t lo = vo < u ? vo : u, hi = vo < u ? u : vo;
-    const uint32_t dout = vo > u ? 1u : 0u;
-    const int s = 2 * (int)blk + (int)(wd >> 1);
-    const bool out = (wd & 1u) == dout;
-    double lam;
-    if (out) {
-        lam = hop_mean(n.d[(n.C > 1 ? s_c[i] : 0) * n.S + s], s_x[s][i], wq, sv.tau, a.n_critical);
-    } else {
-        const int cu = n.C > 1 ? a.comp[ri.comp_off + u] : 0;
-        lam = hop_mean(n.d[cu * n.S + s], X[ri.x_off + (uint64_t)s * ri.N + u], wq, sv.tau, a.n_critical);
-    }
-    if (!(lam < kPoissonInvMax)) return true;
-    const U4 o = philox10(lo, hi, sv.step, ctr_word3(sv.real, sv.attempt, kHop, 0u) | blk, k0, k1);
-    const long long k = inversion_search(lam, pick(o, wd));
-    if (k == 0) return false;
-    if (out) {
-        fire += (unsigned long long)k;
-        atomicAdd(&s_del[s][i], (unsigned long long)(-k));
-    } else {
-        atomicAdd(&s_del[s][i], (unsigned long long)k);
-    }
-    return false;
+    const QueuedHop h = queued_hop<MS>(a, n, ri, X, v0, sv, s_x, s_rp, s_c, code);
+    const U4 o = philox10(v0 + h.i, h.u, sv.step, ctr_word3(sv.real, sv.attempt, kHop, 0u) | h.blk, k0, k1);
+    const long long k = inversion_search(h.lam, pick(o, h.wd));
+    if (k == 0) return;
+    fire += (unsigned long long)k;
+    const long long b = h.fwd ? -k : k;  // balance of v0 + i
+    atomicAdd(&s_del[h.s][h.i], (unsigned long long)b);
+    add_balance(a, ri, s_del, v0, h.s, h.u, -b);
 }
 
-// The leap of one tile; false (nothing written) when the tile is left to
-// phase_b_redo_kernel.
-template <int MS, int MR, bool FULL>
-__device__ __forceinline__ bool leap_tile_edges(const KernelArgs& a, const NetView& n, const RealInfo& ri,
-                                                const uint32_t* X, uint32_t* Xn, uint32_t v0,
-                                                const StepView& sv, uint32_t (*s_x)[kTile],
-                                                unsigned long long (*s_del)[kTile], uint32_t* s_rp,
-                                                int32_t* s_c, uint32_t* s_q, uint32_t* s_qn,
-                                                unsigned long long& fire, uint32_t& bad, uint32_t& viol) {
+// The draws of one tile into the global balances.  Returns false when the
+// tile's open draws are left to phase_b_redo_kernel (its settled draws are
+// applied either way).
+template <int MS, int MR, bool REDO>
+__device__ __forceinline__ bool leap_tile_fwd(const KernelArgs& a, const NetView& n, const RealInfo& ri,
+                                              const uint32_t* X, uint32_t v0, const StepView& sv,
+                                              uint32_t (*s_x)[kTile], unsigned long long (*s_del)[kTile],
+                                              uint32_t* s_rp, int32_t* s_c, uint32_t* s_q, uint32_t* s_qn,
+                                              unsigned long long& fire) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t N = ri.N;
     const uint32_t v = v0 + lane;
@@ -968,35 +986,65 @@ __device__ __forceinline__ bool leap_tile_edges(const KernelArgs& a, const NetVi
         s_del[s][lane] = 0ull;
     }
     s_c[lane] = (n.C > 1 && active) ? a.comp[ri.comp_off + v] : 0;
-    s_rp[lane] = a.rp[ri.rp_off + min(v, N)];
+    s_rp[lane] = a.frp[ri.rp_off + min(v, N)];
     if (lane == 0) {
-        s_rp[kTile] = a.rp[ri.rp_off + min(v0 + (uint32_t)kTile, N)];
-        *s_qn = 0u;
+        s_rp[kTile] = a.frp[ri.rp_off + min(v0 + (uint32_t)kTile, N)];
+        if (!REDO) *s_qn = 0u;
     }
     lds_wave_sync();
-    // hop codes hold the edge offset in 27 bits
-    if (!FULL && wave_uniform(s_rp[kTile] - s_rp[0]) >= (1u << 27)) return false;
-    walk_tile_draws<MS, MR, FULL>(a, n, ri, X, v0, sv, s_x, s_del, s_rp, s_c, s_q, s_qn, fire);
+    walk_fwd_draws<MS, MR, REDO>(a, n, ri, X, v0, sv, s_x, s_del, s_rp, s_c, s_q, s_qn, fire);
     lds_wave_sync();
-    if constexpr (!FULL) {
+    bool complete = true;
+    if constexpr (!REDO) {
         const uint32_t cnt = wave_uniform(*s_qn);
-        if (cnt > kQueue) return false;
-        bool ptrs = false;
-        for (uint32_t q = lane; q < cnt; q += (uint32_t)kTile)
-            ptrs = drain_draw<MS>(a, n, ri, X, v0, sv, s_x, s_del, s_rp, s_c, s_q[q], fire) || ptrs;
-        if (__ballot(ptrs) != 0ull) return false;
-        lds_wave_sync();
+        complete = cnt <= kQueue;
+        if (complete) {
+            // a queued hop whose double mean is >= 10 needs PTRS: nothing
+            // queued is drawn then, the redo pass draws all of it
+            bool big = false;
+            for (uint32_t q = lane; q < cnt; q += (uint32_t)kTile) {
+                const uint32_t code = s_q[q];
+                if (!(code & kQueueReact))
+                    big = big || !(queued_hop<MS>(a, n, ri, X, v0, sv, s_x, s_rp, s_c, code).lam < kPoissonInvMax);
+            }
+            complete = __ballot(big) == 0ull;
+        }
+        if (complete) {
+            for (uint32_t q = lane; q < cnt; q += (uint32_t)kTile)
+                drain_fwd_draw<MS>(a, n, ri, X, v0, sv, s_x, s_del, s_rp, s_c, s_q[q], fire);
+            lds_wave_sync();
+        }
     }
-    if (!active) return true;
+    if (active) {
+#pragma unroll
+        for (int s = 0; s < MS; ++s)
+            if (s < n.S && s_del[s][lane] != 0ull) atomicAdd(&a.gdelta[ri.x_off + (uint64_t)s * N + v], s_del[s][lane]);
+    }
+    return complete;
+}
 
+// The end of the leap pass for one tile: new counts = counts + global
+// balances (which are cleared again), the critical event, the negativity
+// and post-leap checks, and the tile's statistics.
+template <int MS>
+__device__ __forceinline__ void finish_tile(const KernelArgs& a, const NetView& n, const RealInfo& ri,
+                                            const uint32_t* X, uint32_t* Xn, uint32_t v0, const StepView& sv,
+                                            unsigned long long& fire, uint32_t& bad, uint32_t& viol) {
+    const uint32_t v = v0 + (threadIdx.x & 63u);
+    if (v >= ri.N) return;
     uint32_t x[MS];
     long long nx[MS];
 #pragma unroll
     for (int s = 0; s < MS; ++s) {
-        x[s] = s_x[s][lane];
-        nx[s] = (long long)x[s] + (long long)s_del[s][lane];
+        x[s] = 0u;
+        nx[s] = 0;
+        if (s >= n.S) continue;
+        const uint64_t at = ri.x_off + (uint64_t)s * ri.N + v;
+        x[s] = X[at];
+        const unsigned long long d = a.gdelta[at];
+        if (d != 0ull) a.gdelta[at] = 0ull;
+        nx[s] = (long long)x[s] + (long long)d;
     }
-    // the critical event (one per global step, in the argmin-tau'' subvolume)
     if (sv.crit) {
         if (v == sv.vstar) {
             if (sv.ev_kind == 1) {
@@ -1022,11 +1070,10 @@ __device__ __forceinline__ bool leap_tile_edges(const KernelArgs& a, const NetVi
     for (int s = 0; s < MS; ++s) {
         if (s >= n.S) break;
         neg = neg || nx[s] < 0 || nx[s] > 0xFFFFFFFFll;
-        Xn[ri.x_off + (uint64_t)s * N + v] = (uint32_t)nx[s];
+        Xn[ri.x_off + (uint64_t)s * ri.N + v] = (uint32_t)nx[s];
     }
     bad = neg ? 1u : 0u;
     viol = leap_violation<MS>(n.hor, n.mult, n.S, x, nx, a.epsilon, a.theta);
-    return true;
 }
 
 __device__ __forceinline__ void reduce_partB(unsigned long long& fire, uint32_t& bad, uint32_t& viol) {
@@ -1093,19 +1140,7 @@ __device__ void finalize_B(const KernelArgs& a, const RealInfo& ri, uint32_t r, 
     a.ctl[r] = c;
 }
 
-// Phase B of one tile.  EP: edge-parallel tile (every realization of the
-// context is symmetric; FULL = complete sampler, the redo pass); otherwise
-// one lane walks one subvolume's out- and in-edges.
-template <int MS, int MR, bool EP, bool FULL>
-__device__ __forceinline__ void phase_b_tile(const KernelArgs& a, const NetView& n, uint32_t t,
-                                             uint32_t (*s_x)[kTile], unsigned long long (*s_del)[kTile],
-                                             uint32_t* s_rp, int32_t* s_c, uint32_t* s_q, uint32_t* s_qn) {
-    const uint32_t lane = threadIdx.x & 63u;
-    const TileEnt te = a.tiles[t];
-    const uint32_t r = wave_uniform(te.r);
-    const RealInfo ri = a.info[r];
-    const RealCtl* cp = &a.ctl[r];
-    if (wave_uniform(cp->done)) return;
+__device__ __forceinline__ StepView step_view(const RealInfo& ri, const RealCtl* cp) {
     StepView sv;
     sv.tau = cp->tau;
     sv.step = wave_uniform(cp->step);
@@ -1116,20 +1151,41 @@ __device__ __forceinline__ void phase_b_tile(const KernelArgs& a, const NetView&
     sv.ev_idx = (int)wave_uniform((uint32_t)cp->ev_idx);
     sv.vstar = wave_uniform(cp->vstar);
     sv.ev_dst = wave_uniform(cp->ev_dst);
+    return sv;
+}
+
+// Phase B of one tile.  EP: the draws of an edge-parallel tile into the
+// global balances (FULL = complete sampler, the redo pass); the tile's
+// firings go to its partial and phase_b_finish_kernel completes the leap.
+// Otherwise one lane walks one subvolume's out- and in-edges and finishes it.
+template <int MS, int MR, bool EP, bool FULL>
+__device__ __forceinline__ void phase_b_tile(const KernelArgs& a, const NetView& n, uint32_t t,
+                                             uint32_t (*s_x)[kTile], unsigned long long (*s_del)[kTile],
+                                             uint32_t* s_rp, int32_t* s_c, uint32_t* s_q, uint32_t* s_qn) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const TileEnt te = a.tiles[t];
+    const uint32_t r = wave_uniform(te.r);
+    const RealInfo ri = a.info[r];
+    const RealCtl* cp = &a.ctl[r];
+    if (wave_uniform(cp->done)) return;
+    const StepView sv = step_view(ri, cp);
     const uint32_t cur = wave_uniform(cp->cur);
     const uint32_t* X = cur ? a.x1 : a.x0;
     uint32_t* Xn = cur ? a.x0 : a.x1;
     unsigned long long fire = 0;
     uint32_t bad = 0, viol = 0;
     if constexpr (EP) {
-        if (!leap_tile_edges<MS, MR, FULL>(a, n, ri, X, Xn, te.v0, sv, s_x, s_del, s_rp, s_c, s_q, s_qn, fire, bad,
-                                           viol)) {
-            if (lane == 0) {
+        const bool complete = leap_tile_fwd<MS, MR, FULL>(a, n, ri, X, te.v0, sv, s_x, s_del, s_rp, s_c, s_q, s_qn, fire);
+        reduce_partB(fire, bad, viol);
+        if (lane == 0) {
+            if (!complete) {
                 const uint32_t slot = atomicAdd(a.redo_count, 1u);
                 if (slot < a.ntiles) a.redo_tiles[slot] = t;  // one entry per tile per pass
             }
-            return;
+            // the redo pass adds the firings of the open draws
+            a.partB_tile[t] = PartB{(FULL ? a.partB_tile[t].fire : 0ull) + fire, 0u, 0u};
         }
+        return;
     } else {
         const uint32_t v = te.v0 + lane;
         if (v < ri.N) bad = leap_subvolume<MS, MR>(a, n, ri, X, Xn, v, sv, fire, viol) ? 1u : 0u;
@@ -1160,8 +1216,8 @@ void phase_b_kernel(KernelArgs a) {
     }
 }
 
-// Tiles phase_b_kernel left on the redo list, drawn with the complete
-// sampler.  The list is emptied by decide_kernel before each leap pass.
+// Tiles phase_b_kernel left on the redo list: their open draws, drawn with
+// the complete sampler.  The list is emptied by decide_kernel before each leap pass.
 template <int MS, int MR, bool MC>
 __global__ __launch_bounds__(256) void phase_b_redo_kernel(KernelArgs a) {
     const NetView n = net_view<MC>(a);
@@ -1171,12 +1227,35 @@ __global__ __launch_bounds__(256) void phase_b_redo_kernel(KernelArgs a) {
     __shared__ unsigned long long s_del[4][MS][kTile];
     __shared__ uint32_t s_rp[4][kTile + 1];
     __shared__ int32_t s_c[4][kTile];
-    __shared__ uint32_t s_qn[4];
     const uint32_t cnt = min(wave_uniform(*a.redo_count), a.ntiles);
     if (blockIdx.x == 0 && threadIdx.x == 0 && cnt != 0u) *a.redo_total += cnt;
     for (uint32_t q = blockIdx.x * wpb + wv; q < cnt; q += gridDim.x * wpb)
         phase_b_tile<MS, MR, true, true>(a, n, wave_uniform(a.redo_tiles[q]), s_x[wv], s_del[wv], s_rp[wv], s_c[wv],
-                                         nullptr, &s_qn[wv]);
+                                         nullptr, nullptr);
+}
+
+// The end of the edge-parallel leap pass: every tile adds the global
+// balances to its counts (see finish_tile) after all draws are in.
+template <int MS, int MR, bool MC>
+__global__ __launch_bounds__(256) void phase_b_finish_kernel(KernelArgs a) {
+    const NetView n = net_view<MC>(a);
+    const uint32_t lane = threadIdx.x & 63u;
+    for (uint32_t it = 0;; ++it) {
+        const uint32_t t = tile_at(it, a.ntiles, a.waves_per_block);
+        if (t == 0xFFFFFFFFu) break;
+        const TileEnt te = a.tiles[t];
+        const uint32_t r = wave_uniform(te.r);
+        const RealInfo ri = a.info[r];
+        const RealCtl* cp = &a.ctl[r];
+        if (wave_uniform(cp->done)) continue;
+        const StepView sv = step_view(ri, cp);
+        const uint32_t cur = wave_uniform(cp->cur);
+        unsigned long long fire = 0;
+        uint32_t bad = 0, viol = 0;
+        finish_tile<MS>(a, n, ri, cur ? a.x1 : a.x0, cur ? a.x0 : a.x1, te.v0, sv, fire, bad, viol);
+        reduce_partB(fire, bad, viol);
+        if (lane == 0) a.partB_tile[t] = PartB{a.partB_tile[t].fire + fire, bad, viol};
+    }
 }
 
 // One block per realization: sum of firings / any negative over the tile
@@ -1235,7 +1314,9 @@ __global__ __launch_bounds__(1024) void commit_kernel(KernelArgs a) {
     template __global__ void phase_b_kernel<MS, MR, true, false>(KernelArgs); \
     template __global__ void phase_b_kernel<MS, MR, true, true>(KernelArgs);  \
     template __global__ void phase_b_redo_kernel<MS, MR, false>(KernelArgs);  \
-    template __global__ void phase_b_redo_kernel<MS, MR, true>(KernelArgs);
+    template __global__ void phase_b_redo_kernel<MS, MR, true>(KernelArgs);   \
+    template __global__ void phase_b_finish_kernel<MS, MR, false>(KernelArgs); \
+    template __global__ void phase_b_finish_kernel<MS, MR, true>(KernelArgs);
 PLIS_INST(1, 4)
 PLIS_INST(3, 4)
 PLIS_INST(5, 8)

@@ -41,6 +41,7 @@ struct RealInfo {
     uint64_t rp_off;     // out row_ptr base (N+1 entries)
     uint64_t irp_off;    // in row_ptr base
     uint64_t comp_off;   // compartment ids base (if C > 1)
+    uint64_t fe_off;     // forward-CSR edge base (entries v -> u with u > v; symmetric only)
     uint32_t N, id;
     uint32_t tile_begin, ntiles;
     uint32_t flags;      // bit0: symmetric (in-CSR aliases out-CSR), bit1: weighted
@@ -102,6 +103,12 @@ struct KernelArgs {
     uint32_t* ndone;
     uint32_t* redo_count;    // tiles phase B left to the redo pass (reset by decide)
     uint32_t* redo_tiles;
+    // forward CSR (row_ptr layout of rp; entries u > v) and the leap pass's
+    // global per-subvolume balances (counts layout, zero between passes)
+    const uint32_t* frp;
+    const uint32_t* fcol;
+    const float* fw;
+    unsigned long long* gdelta;
     unsigned long long* redo_total;  // tiles redone since the context was made
     double epsilon;
     int32_t n_critical;

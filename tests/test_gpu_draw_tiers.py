@@ -2,8 +2,10 @@
 
 phase_b_kernel settles draws with the zero shortcut and the float squeeze,
 queues the rest per wavefront (at most 128) and draws them with the double
-search after the walk; a tile with a mean >= 10 draw or a full queue is
-redone from scratch by phase_b_redo_kernel with the complete sampler.  These
+search after the walk; a tile with a mean >= 10 draw or a full queue keeps
+its settled draws and phase_b_redo_kernel draws its open ones with the
+complete sampler.  Each undirected edge belongs to the tile of its lower
+end, which draws both of its hop directions.  These
 cases drive each branch on purpose -- draws queued and drained, queues that
 overflow on some tiles only, every tile redone -- predict on the host which
 tiles the leap pass must hand to the redo pass (the same draws, rebuilt from
@@ -56,7 +58,7 @@ def _tune_eps(net, topo, x0, lam_of_tau, lo_t, hi_t):
 
 def _predict_step0(net, topo, x0, eps, seed=1, nc=10):
     """Per tile of the first leap pass: number of draws the fast path leaves
-    undecided (mean < 10) and whether a mean >= 10 draw occurs.  Unweighted
+    open (mean < 10) and whether a mean >= 10 draw occurs.  Unweighted
     symmetric topology, one compartment, realization 0, step 0, attempt 0."""
     S, R, N = net["S"], net["R"], topo["N"]
     x = x0.reshape(S, N).astype(np.int64)
@@ -84,14 +86,14 @@ def _predict_step0(net, topo, x0, eps, seed=1, nc=10):
         t = v // 64
         for q in range(rp[v], rp[v + 1]):
             u = col[q]
-            lo, hi = min(v, u), max(v, u)
-            dout = 1 if v > u else 0
+            if u < v:
+                continue  # each undirected edge is drawn by the tile of its lower end
             for blk in range((S + 1) // 2):
-                o = ffi.philox((lo, hi, 0, (P_HOP << 7) | blk), key)
+                o = ffi.philox((v, u, 0, (P_HOP << 7) | blk), key)
                 for s in (2 * blk, 2 * blk + 1):
                     if s >= S:
                         continue
-                    for src, dirbit in ((v, dout), (u, 1 - dout)):
+                    for src, dirbit in ((v, 0), (u, 1)):
                         xs = x[s, src]
                         if d[s] > 0 and xs >= nc and xs > 0:
                             draw(t, tau * ((d[s] * float(xs)) * 1.0), o[2 * (s & 1) + dirbit])
@@ -136,7 +138,7 @@ def _first_steps_match(net, topo, x0, eps, steps, expect_redo, seed=1):
 
 def test_queued_draws_drained_in_tile():
     """Conversion X0 <-> X1 plus diffusion, largest mean just under 10: every
-    tile queues a few undecided draws (hops and reactions), none overflows."""
+    tile queues a few open draws (hops and reactions), none overflows."""
     net = W.network(["X0", "X1"], [({"X0": 1}, {"X1": 1}), ({"X1": 1}, {"X0": 1})], [1.0, 1.0],
                     [1.0 / 6.0, 1.0 / 6.0])
     topo = W.cubic_grid(8, 8, 8, periodic=True)
@@ -148,11 +150,12 @@ def test_queued_draws_drained_in_tile():
 
 
 def test_queue_overflow_sends_some_tiles_to_redo():
-    """Eight diffusing species, hop means 8-9.7: the undecided draws of some
-    tiles exceed the queue, those tiles (and only those) are redone."""
+    """Eight diffusing species, hop means 8-9.7, degrees 8-16: the open
+    draws of some tiles exceed the queue, those tiles (and only those) are
+    redone."""
     S = 8
     net = W.network([f"X{s}" for s in range(S)], [({}, {"X0": 1})], [0.0], [1.0] * S)
-    topo = W.cubic_grid(8, 8, 8, periodic=True)
+    topo = W.random_mesh(8, 8, 8, 7, dmin=8, dmax=16, wlo=1.0, whi=1.0)
     x0 = W.poisson_state(S, topo["N"], 2000.0, 3)
     eps = _tune_eps(net, topo, x0, lambda tau: tau * float(x0.max()), 9.6, 9.65)
     und, big = _predict_step0(net, topo, x0, eps)
