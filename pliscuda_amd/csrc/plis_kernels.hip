@@ -117,7 +117,17 @@ __device__ __forceinline__ double g_cao(int hor, int mult, uint32_t xs) {
 }
 
 // Post-leap check (Anderson 2008 form of Cao's leap condition): a reactant
-// species whose realized change exceeds theta * max(eps x / g, 1).
+// species whose realized change exceeds theta * max(eps x / g, 1) (species
+// of highest order 0 are free).
+__device__ __forceinline__ bool species_violates(int hor, int mult, uint32_t x, long long nx, double eps,
+                                                 double theta) {
+    if (hor == 0) return false;
+    const double dx = __builtin_fabs((double)nx - (double)x);
+    const double bound = theta * fmax((eps * (double)x) / g_cao(hor, mult, x), 1.0);
+    return dx > bound;
+}
+
+// Any violating species at one subvolume.
 template <int MS>
 __device__ __forceinline__ uint32_t leap_violation(const int32_t* hor, const int32_t* mult, int S,
                                                    const uint32_t (&x)[MS], const long long (&nx)[MS],
@@ -125,10 +135,8 @@ __device__ __forceinline__ uint32_t leap_violation(const int32_t* hor, const int
     uint32_t bad = 0;
 #pragma unroll
     for (int s = 0; s < MS; ++s) {
-        if (s >= S || hor[s] == 0) continue;
-        const double dx = __builtin_fabs((double)nx[s] - (double)x[s]);
-        const double bound = theta * fmax((eps * (double)x[s]) / g_cao(hor[s], mult[s], x[s]), 1.0);
-        if (dx > bound) bad = 1u;
+        if (s >= S) break;
+        if (species_violates(hor[s], mult[s], x[s], nx[s], eps, theta)) bad = 1u;
     }
     return bad;
 }
@@ -183,7 +191,8 @@ __device__ __forceinline__ void decide(RealCtl& c) {
 
 // Critical event at v*: a critical reaction or a critical hop, chosen with
 // probability proportional to its propensity; a hop's edge is chosen in
-// proportion to its weight.  (The reference picks a uniform slot among 6
+// proportion to its weight.  Run by a whole wavefront (every value is
+// wave-uniform).  (The reference picks a uniform slot among 6
 // including self-padding, PLISCuda/src/leap.cu:180-194, ssa.cu:88-106.)
 template <int MS, int MR>
 __device__ void choose_event(const KernelArgs& a, const NetView& n, const RealInfo& ri, RealCtl& c,
@@ -238,13 +247,26 @@ __device__ void choose_event(const KernelArgs& a, const NetView& n, const RealIn
     c.ev_idx = idx;
     c.ev_dst = v;
     if (kind == 2) {
+        // the wave loads up to 64 edges of v's row at once; the weights are
+        // then summed one edge at a time in CSR order
+        const uint32_t lane = threadIdx.x & 63u;
         const double tw = uniform01(o.y) * W;
         double accw = 0.0;
         uint32_t dst = v;
-        for (uint32_t q = rb; q < re; ++q) {
-            accw += weighted ? (double)a.w[ri.e_off + q] : 1.0;
-            dst = a.col[ri.e_off + q];
-            if (tw < accw) break;
+        bool found = false;
+        for (uint32_t q0 = rb; q0 < re && !found; q0 += 64u) {
+            const uint32_t q = q0 + lane;
+            const float wl = q < re ? (weighted ? a.w[ri.e_off + q] : 1.0f) : 0.0f;
+            const uint32_t cl = q < re ? a.col[ri.e_off + q] : v;
+            const uint32_t cnt = min(64u, re - q0);
+            for (uint32_t i = 0; i < cnt; ++i) {
+                accw += (double)__shfl(wl, (int)i);
+                dst = (uint32_t)__shfl((int)cl, (int)i);
+                if (tw < accw) {
+                    found = true;
+                    break;
+                }
+            }
         }
         c.ev_dst = dst;
     }
@@ -388,12 +410,15 @@ __device__ __forceinline__ void reduce_partA(double& tp, double& t2, uint32_t& v
 template <int MS, int MR>
 __device__ void finalize_A(const KernelArgs& a, const NetView& n, const RealInfo& ri, uint32_t r,
                            double tp, double t2, uint32_t v2) {
+    const bool lead = (threadIdx.x & 63u) == 0u;
     RealCtl c = a.ctl[r];
     if (tp == kInf && t2 == kInf) {
         c.status = kAbsorbed;
         c.done = 1u;
-        a.ctl[r] = c;
-        atomicAdd(a.ndone, 1u);
+        if (lead) {
+            a.ctl[r] = c;
+            atomicAdd(a.ndone, 1u);
+        }
         return;
     }
     c.tau_p = tp;
@@ -405,7 +430,7 @@ __device__ void finalize_A(const KernelArgs& a, const NetView& n, const RealInfo
     c.ev_dst = v2;
     decide(c);
     if (t2 < kInf) choose_event<MS, MR>(a, n, ri, c, c.cur ? a.x1 : a.x0);
-    a.ctl[r] = c;
+    if (lead) a.ctl[r] = c;
 }
 
 template <int MS, int MR, bool MC>
@@ -492,7 +517,7 @@ __global__ __launch_bounds__(decide_threads<MS>()) void decide_kernel(KernelArgs
         s_v2[wv] = v2;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (wv == 0) {  // the first wavefront, every lane alike
         for (uint32_t w = 1; w < (blockDim.x >> 6); ++w) {
             tp = fmin(tp, s_tp[w]);
             if (s_t2[w] < t2 || (s_t2[w] == t2 && s_v2[w] < v2)) {
@@ -1032,48 +1057,31 @@ __device__ __forceinline__ void finish_tile(const KernelArgs& a, const NetView& 
                                             unsigned long long& fire, uint32_t& bad, uint32_t& viol) {
     const uint32_t v = v0 + (threadIdx.x & 63u);
     if (v >= ri.N) return;
-    uint32_t x[MS];
-    long long nx[MS];
-#pragma unroll
-    for (int s = 0; s < MS; ++s) {
-        x[s] = 0u;
-        nx[s] = 0;
-        if (s >= n.S) continue;
-        const uint64_t at = ri.x_off + (uint64_t)s * ri.N + v;
-        x[s] = X[at];
-        const unsigned long long d = a.gdelta[at];
-        if (d != 0ull) a.gdelta[at] = 0ull;
-        nx[s] = (long long)x[s] + (long long)d;
-    }
-    if (sv.crit) {
-        if (v == sv.vstar) {
-            if (sv.ev_kind == 1) {
-#pragma unroll
-                for (int s = 0; s < MS; ++s)
-                    if (s < n.S) nx[s] += n.nu[s * n.R + sv.ev_idx];
-                fire += 1ull;
-            } else if (sv.ev_kind == 2) {
-#pragma unroll
-                for (int s = 0; s < MS; ++s)
-                    if (s == sv.ev_idx) nx[s] -= 1;
-                fire += 1ull;
-            }
-        }
-        if (sv.ev_kind == 2 && v == sv.ev_dst && sv.ev_dst != sv.vstar) {
-#pragma unroll
-            for (int s = 0; s < MS; ++s)
-                if (s == sv.ev_idx) nx[s] += 1;
-        }
-    }
+    // one species at a time, few registers live
+    const bool at_star = sv.crit && v == sv.vstar && (sv.ev_kind == 1 || sv.ev_kind == 2);
+    const bool at_dst = sv.crit && sv.ev_kind == 2 && v == sv.ev_dst && sv.ev_dst != sv.vstar;
+    if (at_star) fire += 1ull;
     bool neg = false;
+    uint32_t vio = 0u;
 #pragma unroll
     for (int s = 0; s < MS; ++s) {
         if (s >= n.S) break;
-        neg = neg || nx[s] < 0 || nx[s] > 0xFFFFFFFFll;
-        Xn[ri.x_off + (uint64_t)s * ri.N + v] = (uint32_t)nx[s];
+        const uint64_t at = ri.x_off + (uint64_t)s * ri.N + v;
+        const uint32_t x = X[at];
+        const unsigned long long d = a.gdelta[at];
+        if (d != 0ull) a.gdelta[at] = 0ull;
+        long long nx = (long long)x + (long long)d;
+        if (at_star) {
+            if (sv.ev_kind == 1) nx += n.nu[s * n.R + sv.ev_idx];
+            else if (s == sv.ev_idx) nx -= 1;
+        }
+        if (at_dst && s == sv.ev_idx) nx += 1;
+        neg = neg || nx < 0 || nx > 0xFFFFFFFFll;
+        Xn[at] = (uint32_t)nx;
+        if (species_violates(n.hor[s], n.mult[s], x, nx, a.epsilon, a.theta)) vio = 1u;
     }
     bad = neg ? 1u : 0u;
-    viol = leap_violation<MS>(n.hor, n.mult, n.S, x, nx, a.epsilon, a.theta);
+    viol = vio;
 }
 
 __device__ __forceinline__ void reduce_partB(unsigned long long& fire, uint32_t& bad, uint32_t& viol) {
