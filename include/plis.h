@@ -128,6 +128,14 @@ int plis_iterate(plis_ctx *ctx, uint32_t n);
  * the decide kernel, the leap pass and the commit kernel (out[1..4]). */
 int plis_profile(plis_ctx *ctx, uint32_t n, double *out);
 
+/* As plis_profile, with the leap pass split by kernel: out[0..4] as there,
+ * out[5] = phase_b_kernel, out[6] = phase_b_redo_kernel, out[7] =
+ * phase_b_finish_kernel (summed ms).  A context with the per-subvolume leap
+ * kernel (some topology not symmetric) reports it in out[5] and 0 in
+ * out[6..7].  Diagnostics of this implementation; no counterpart in the
+ * reference. */
+int plis_profile_kernels(plis_ctx *ctx, uint32_t n, double *out);
+
 int plis_set_state(plis_ctx *ctx, int32_t r, const uint32_t *x);
 int plis_get_state(plis_ctx *ctx, int32_t r, uint32_t *x);
 int plis_get_stats(plis_ctx *ctx, int32_t r, plis_stats *st);

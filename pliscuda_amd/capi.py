@@ -19,6 +19,7 @@ ERRORS = {-1: "PLIS_ERR_ARG", -2: "PLIS_ERR_HIP", -3: "PLIS_ERR_LEAPBACK", -4: "
 
 # every entry point declared in include/plis.h
 EXPORTS = ("plis_create", "plis_run", "plis_run_until", "plis_arm", "plis_iterate", "plis_profile",
+           "plis_profile_kernels",
            "plis_set_state", "plis_get_state", "plis_get_stats", "plis_sync", "plis_debug_tau",
            "plis_launch_shape", "plis_redo_tiles", "plis_checkpoint_size", "plis_checkpoint", "plis_restore", "plis_autotune", "plis_destroy", "plis_last_error",
            "plis_device_count", "plis_mlp_set", "plis_mlp_get", "plis_mlp_forward",
@@ -78,6 +79,7 @@ def _declare(L):
     L.plis_arm.argtypes = [vp, C.c_double, C.c_double, C.c_uint64]
     L.plis_iterate.argtypes = [vp, C.c_uint32]
     L.plis_profile.argtypes = [vp, C.c_uint32, P(C.c_double)]
+    L.plis_profile_kernels.argtypes = [vp, C.c_uint32, P(C.c_double)]
     L.plis_set_state.argtypes = [vp, C.c_int32, P(C.c_uint32)]
     L.plis_get_state.argtypes = [vp, C.c_int32, P(C.c_uint32)]
     L.plis_get_stats.argtypes = [vp, C.c_int32, P(Stats)]
@@ -191,6 +193,12 @@ class Context:
         """[total, tau, decide, leap, commit] milliseconds summed over n iterations."""
         out = (C.c_double * 5)()
         self._chk(lib().plis_profile(self.h, n, out), "plis_profile")
+        return list(out)
+
+    def profile_kernels(self, n):
+        """profile() plus the leap pass by kernel: [..., phase_b, redo, finish]."""
+        out = (C.c_double * 8)()
+        self._chk(lib().plis_profile_kernels(self.h, n, out), "plis_profile_kernels")
         return list(out)
 
     def sync(self):

@@ -59,6 +59,7 @@ struct plis_ctx {
     uint32_t wpb = 4, blocks = 8;
     int use_graph = 1;
     bool edge_parallel = false;  // every realization symmetric, tiles < 2^27 forward entries
+    hipEvent_t* split_ev = nullptr;  // profiling: events between the leap pass's kernels
     hipGraphExec_t graph = nullptr;
     uint32_t graph_iters = 0;
     std::vector<void*> allocs;
@@ -315,11 +316,15 @@ static void launch_variant(plis_ctx* c, int which, const KernelArgs& a) {
                 if (c->edge_parallel) {
                     if (mc) {
                         hipLaunchKernelGGL((phase_b_kernel<MS, MR, true, true>), grid, block, 0, c->stream, a);
+                        if (c->split_ev) (void)hipEventRecord(c->split_ev[0], c->stream);
                         hipLaunchKernelGGL((phase_b_redo_kernel<MS, MR, true>), grid, block, 0, c->stream, a);
+                        if (c->split_ev) (void)hipEventRecord(c->split_ev[1], c->stream);
                         hipLaunchKernelGGL((phase_b_finish_kernel<MS, MR, true>), grid, block, 0, c->stream, a);
                     } else {
                         hipLaunchKernelGGL((phase_b_kernel<MS, MR, true, false>), grid, block, 0, c->stream, a);
+                        if (c->split_ev) (void)hipEventRecord(c->split_ev[0], c->stream);
                         hipLaunchKernelGGL((phase_b_redo_kernel<MS, MR, false>), grid, block, 0, c->stream, a);
+                        if (c->split_ev) (void)hipEventRecord(c->split_ev[1], c->stream);
                         hipLaunchKernelGGL((phase_b_finish_kernel<MS, MR, false>), grid, block, 0, c->stream, a);
                     }
                     break;
@@ -700,34 +705,59 @@ extern "C" int plis_iterate(plis_ctx* c, uint32_t n) {
     return PLIS_OK;
 }
 
-extern "C" int plis_profile(plis_ctx* c, uint32_t n, double* out) {
+extern "C" int plis_profile_kernels(plis_ctx* c, uint32_t n, double* out) {
     if (!c || !out) return PLIS_ERR_ARG;
-    std::vector<hipEvent_t> ev(8 * (size_t)n + 2);
+    // per iteration: before/after each of the 4 phases, and 2 events
+    // between the leap pass's kernels
+    std::vector<hipEvent_t> ev(10 * (size_t)n + 2);
     for (auto& e : ev) HIPCHK(c, hipEventCreate(&e));
     const KernelArgs a = make_args(c);
     HIPCHK(c, hipEventRecord(ev[0], c->stream));
     for (uint32_t i = 0; i < n; ++i)
         for (int k = 0; k < 4; ++k) {
-            HIPCHK(c, hipEventRecord(ev[1 + 8 * i + 2 * k], c->stream));
+            hipEvent_t* e = &ev[1 + 10 * (size_t)i];
+            HIPCHK(c, hipEventRecord(e[2 * k], c->stream));
+            c->split_ev = k == 2 ? &e[8] : nullptr;
             launch_kernel(c, k, a);
-            HIPCHK(c, hipEventRecord(ev[2 + 8 * i + 2 * k], c->stream));
+            c->split_ev = nullptr;
+            HIPCHK(c, hipEventRecord(e[2 * k + 1], c->stream));
         }
-    HIPCHK(c, hipEventRecord(ev[8 * n + 1], c->stream));
+    HIPCHK(c, hipEventRecord(ev[10 * (size_t)n + 1], c->stream));
     HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipEventSynchronize(ev[8 * n + 1]));
+    HIPCHK(c, hipEventSynchronize(ev[10 * (size_t)n + 1]));
     float tot = 0.f;
-    double t[4] = {0, 0, 0, 0};
-    HIPCHK(c, hipEventElapsedTime(&tot, ev[0], ev[8 * n + 1]));
-    for (uint32_t i = 0; i < n; ++i)
+    double t[7] = {0, 0, 0, 0, 0, 0, 0};
+    HIPCHK(c, hipEventElapsedTime(&tot, ev[0], ev[10 * (size_t)n + 1]));
+    for (uint32_t i = 0; i < n; ++i) {
+        hipEvent_t* e = &ev[1 + 10 * (size_t)i];
+        float x = 0.f;
         for (int k = 0; k < 4; ++k) {
-            float x = 0.f;
-            HIPCHK(c, hipEventElapsedTime(&x, ev[1 + 8 * i + 2 * k], ev[2 + 8 * i + 2 * k]));
+            HIPCHK(c, hipEventElapsedTime(&x, e[2 * k], e[2 * k + 1]));
             t[k] += x;
         }
+        if (c->edge_parallel) {
+            HIPCHK(c, hipEventElapsedTime(&x, e[4], e[8]));
+            t[4] += x;
+            HIPCHK(c, hipEventElapsedTime(&x, e[8], e[9]));
+            t[5] += x;
+            HIPCHK(c, hipEventElapsedTime(&x, e[9], e[5]));
+            t[6] += x;
+        }
+    }
+    if (!c->edge_parallel) t[4] = t[2];  // one leap kernel
     for (auto& e : ev) (void)hipEventDestroy(e);
     out[0] = tot;
-    for (int k = 0; k < 4; ++k) out[1 + k] = t[k];
+    for (int k = 0; k < 7; ++k) out[1 + k] = t[k];
     return PLIS_OK;
+}
+
+extern "C" int plis_profile(plis_ctx* c, uint32_t n, double* out) {
+    if (!out) return PLIS_ERR_ARG;
+    double o[8];
+    const int rc = plis_profile_kernels(c, n, o);
+    if (rc == PLIS_OK)
+        for (int k = 0; k < 5; ++k) out[k] = o[k];
+    return rc;
 }
 
 extern "C" int plis_sync(plis_ctx* c) {
