@@ -32,9 +32,15 @@ namespace plis {
 
 constexpr double kInf = __builtin_huge_val();
 
+// The network tables are read-only for every kernel: reading them through
+// the constant address space lets the compiler use scalar loads (scalar
+// cache, no vector-memory round trip) wherever the index is wave-uniform.
+template <class T>
+using cptr = const T __attribute__((address_space(4)))*;
+
 struct NetView {
-    const int32_t *nu, *rs, *rr, *nrs, *order, *hor, *mult;
-    const double *k, *d;
+    cptr<int32_t> nu, rs, rr, nrs, order, hor, mult;
+    cptr<double> k, d;
     int S, R, C;
 };
 
@@ -44,15 +50,15 @@ struct NetView {
 template <bool MC = true>
 __device__ __forceinline__ NetView net_view(const KernelArgs& a) {
     NetView n;
-    n.nu = a.net_i + a.nh.off_nu;
-    n.rs = a.net_i + a.nh.off_rs;
-    n.rr = a.net_i + a.nh.off_rr;
-    n.nrs = a.net_i + a.nh.off_nrs;
-    n.order = a.net_i + a.nh.off_order;
-    n.hor = a.net_i + a.nh.off_hor;
-    n.mult = a.net_i + a.nh.off_mult;
-    n.k = a.net_d + a.nh.off_k;
-    n.d = a.net_d + a.nh.off_d;
+    n.nu = (cptr<int32_t>)(a.net_i + a.nh.off_nu);
+    n.rs = (cptr<int32_t>)(a.net_i + a.nh.off_rs);
+    n.rr = (cptr<int32_t>)(a.net_i + a.nh.off_rr);
+    n.nrs = (cptr<int32_t>)(a.net_i + a.nh.off_nrs);
+    n.order = (cptr<int32_t>)(a.net_i + a.nh.off_order);
+    n.hor = (cptr<int32_t>)(a.net_i + a.nh.off_hor);
+    n.mult = (cptr<int32_t>)(a.net_i + a.nh.off_mult);
+    n.k = (cptr<double>)(a.net_d + a.nh.off_k);
+    n.d = (cptr<double>)(a.net_d + a.nh.off_d);
     n.S = a.nh.S;
     n.R = a.nh.R;
     n.C = MC ? a.nh.C : 1;
@@ -129,7 +135,7 @@ __device__ __forceinline__ bool species_violates(int hor, int mult, uint32_t x, 
 
 // Any violating species at one subvolume.
 template <int MS>
-__device__ __forceinline__ uint32_t leap_violation(const int32_t* hor, const int32_t* mult, int S,
+__device__ __forceinline__ uint32_t leap_violation(cptr<int32_t> hor, cptr<int32_t> mult, int S,
                                                    const uint32_t (&x)[MS], const long long (&nx)[MS],
                                                    double eps, double theta) {
     uint32_t bad = 0;
@@ -332,17 +338,16 @@ __device__ __forceinline__ void tau_own(const KernelArgs& a, const NetView& n, c
     }
 }
 
-// In-hops (nu = +1) from one in-neighbour, species whose hops are non-critical.
-template <int MS, typename XU>
-__device__ __forceinline__ void tau_in_edge(const NetView& n, int cu, double wq, int nc, XU xu_of,
-                                            TauAcc<MS>& t) {
+// In-hops (nu = +1) from one in-neighbour, species whose hops are non-critical:
+// du / xu are the neighbour's diffusion constants and counts.
+template <int MS>
+__device__ __forceinline__ void tau_in_edge(int S, const double (&du)[MS], const uint32_t (&xu)[MS], double wq,
+                                            int nc, TauAcc<MS>& t) {
 #pragma unroll
     for (int s = 0; s < MS; ++s) {
-        if (s >= n.S) break;
-        const uint32_t xu = xu_of(s);
-        const double du = n.d[cu * n.S + s];
-        if (!hop_active(du, xu, nc)) continue;
-        const double ain = (du * (double)xu) * wq;
+        if (s >= S) break;
+        if (!hop_active(du[s], xu[s], nc)) continue;
+        const double ain = (du[s] * (double)xu[s]) * wq;
         t.mu[s] += ain;
         t.sig[s] += ain;
     }
@@ -382,12 +387,24 @@ __device__ __forceinline__ void tau_subvolume(const KernelArgs& a, const NetView
     const float* iwp = sym ? a.w + ri.e_off : a.iw + ri.ie_off;
     const uint32_t* irp = sym ? a.rp + ri.rp_off : a.irp + ri.irp_off;
     const uint32_t ib = irp[v], ie = irp[v + 1];
+    // one compartment: the neighbours' diffusion constants are the tile's
+    double du[MS];
+#pragma unroll
+    for (int s = 0; s < MS; ++s) du[s] = n.d[s < n.S ? s : 0];
     for (uint32_t q = ib; q < ie; ++q) {
         const uint32_t u = icol[q];
         const double wq = weighted ? (double)iwp[q] : 1.0;
-        const int cu = n.C > 1 ? a.comp[ri.comp_off + u] : 0;
-        tau_in_edge<MS>(n, cu, wq, a.n_critical,
-                        [&](int s) { return X[ri.x_off + (uint64_t)s * N + u]; }, t);
+        if (n.C > 1) {
+            const int cu = a.comp[ri.comp_off + u];
+#pragma unroll
+            for (int s = 0; s < MS; ++s) du[s] = n.d[cu * n.S + (s < n.S ? s : 0)];
+        }
+        // every count of the neighbour requested at once (clamped species
+        // index: the loads stay in bounds and unconditional)
+        uint32_t xu[MS];
+#pragma unroll
+        for (int s = 0; s < MS; ++s) xu[s] = X[ri.x_off + (uint64_t)(s < n.S ? s : n.S - 1) * N + u];
+        tau_in_edge<MS>(n.S, du, xu, wq, a.n_critical, t);
     }
     tp_out = tau_bound<MS>(a, n, t);
     a0c_out = t.a0c;
