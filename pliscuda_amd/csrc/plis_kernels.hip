@@ -38,6 +38,25 @@ constexpr double kInf = __builtin_huge_val();
 template <class T>
 using cptr = const T __attribute__((address_space(4)))*;
 
+// Per-kernel read-only data (tile table, realization info and control)
+// through the same address space.
+template <class T>
+__device__ __forceinline__ cptr<T> ro(const T* p) {
+    return (cptr<T>)p;
+}
+
+// A whole read-only struct, word by word (merged into wide scalar loads).
+template <class T>
+__device__ __forceinline__ T ld_ro(const T* p) {
+    static_assert(sizeof(T) % 4 == 0, "struct of 32-bit words");
+    T v;
+    uint32_t* d = reinterpret_cast<uint32_t*>(&v);
+    const cptr<uint32_t> w = (cptr<uint32_t>)p;
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 4); ++i) d[i] = w[i];
+    return v;
+}
+
 struct NetView {
     cptr<int32_t> nu, rs, rr, nrs, order, hor, mult;
     cptr<double> k, d;
@@ -457,14 +476,15 @@ __global__ __launch_bounds__(256) void phase_a_kernel(KernelArgs a) {
     for (uint32_t it = 0;; ++it) {
         const uint32_t t = tile_at(it, a.ntiles, a.waves_per_block);
         if (t == 0xFFFFFFFFu) break;
-        const TileEnt te = a.tiles[t];
+        const TileEnt te = ld_ro(a.tiles + t);
         const uint32_t r = wave_uniform(te.r);
-        const RealInfo ri = a.info[r];
-        const uint32_t done = wave_uniform(a.ctl[r].done), retry = wave_uniform(a.ctl[r].retry);
+        const RealInfo ri = ld_ro(a.info + r);
+        const cptr<RealCtl> cp = ro(a.ctl + r);
+        const uint32_t done = wave_uniform(cp->done), retry = wave_uniform(cp->retry);
         const bool dbg = a.dbg_tp != nullptr;
         if (!dbg && (done || retry)) continue;
-        if (!dbg && a.learned_on && !wave_uniform(a.ctl[r].fallback)) continue;  // learned pass
-        const uint32_t cur = wave_uniform(a.ctl[r].cur);
+        if (!dbg && a.learned_on && !wave_uniform(cp->fallback)) continue;  // learned pass
+        const uint32_t cur = wave_uniform(cp->cur);
         const uint32_t* X = cur ? a.x1 : a.x0;
         const uint32_t v = te.v0 + lane;
         double tp = kInf, t2 = kInf, a0c = 0.0;
@@ -472,7 +492,7 @@ __global__ __launch_bounds__(256) void phase_a_kernel(KernelArgs a) {
         if (v < ri.N) {
             tau_subvolume<MS, MR>(a, n, ri, X, v, tp, a0c);
             if (a0c > 0.0) {
-                const uint32_t step = wave_uniform(a.ctl[r].step);
+                const uint32_t step = wave_uniform(cp->step);
                 const U4 o = philox10(v, kSelfId, step, ctr_word3(ri.id, 0u, kTau2, 0u), a.seed_lo,
                                       a.seed_hi);
                 t2 = -dlog(uniform01(o.x)) / a0c;
@@ -1165,7 +1185,7 @@ __device__ void finalize_B(const KernelArgs& a, const RealInfo& ri, uint32_t r, 
     a.ctl[r] = c;
 }
 
-__device__ __forceinline__ StepView step_view(const RealInfo& ri, const RealCtl* cp) {
+__device__ __forceinline__ StepView step_view(const RealInfo& ri, cptr<RealCtl> cp) {
     StepView sv;
     sv.tau = cp->tau;
     sv.step = wave_uniform(cp->step);
@@ -1188,10 +1208,10 @@ __device__ __forceinline__ void phase_b_tile(const KernelArgs& a, const NetView&
                                              uint32_t (*s_x)[kTile], unsigned long long (*s_del)[kTile],
                                              uint32_t* s_rp, int32_t* s_c, uint32_t* s_q, uint32_t* s_qn) {
     const uint32_t lane = threadIdx.x & 63u;
-    const TileEnt te = a.tiles[t];
+    const TileEnt te = ld_ro(a.tiles + t);
     const uint32_t r = wave_uniform(te.r);
-    const RealInfo ri = a.info[r];
-    const RealCtl* cp = &a.ctl[r];
+    const RealInfo ri = ld_ro(a.info + r);
+    const cptr<RealCtl> cp = ro(a.ctl + r);
     if (wave_uniform(cp->done)) return;
     const StepView sv = step_view(ri, cp);
     const uint32_t cur = wave_uniform(cp->cur);
@@ -1268,10 +1288,10 @@ __global__ __launch_bounds__(256) void phase_b_finish_kernel(KernelArgs a) {
     for (uint32_t it = 0;; ++it) {
         const uint32_t t = tile_at(it, a.ntiles, a.waves_per_block);
         if (t == 0xFFFFFFFFu) break;
-        const TileEnt te = a.tiles[t];
+        const TileEnt te = ld_ro(a.tiles + t);
         const uint32_t r = wave_uniform(te.r);
-        const RealInfo ri = a.info[r];
-        const RealCtl* cp = &a.ctl[r];
+        const RealInfo ri = ld_ro(a.info + r);
+        const cptr<RealCtl> cp = ro(a.ctl + r);
         if (wave_uniform(cp->done)) continue;
         const StepView sv = step_view(ri, cp);
         const uint32_t cur = wave_uniform(cp->cur);

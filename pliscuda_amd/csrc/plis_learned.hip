@@ -84,10 +84,10 @@ __global__ __launch_bounds__(256) void learned_tau_kernel(KernelArgs a) {
     for (uint32_t it = 0;; ++it) {
         const uint32_t t = tile_at(it, a.ntiles, a.waves_per_block);
         if (t == 0xFFFFFFFFu) break;
-        const TileEnt te = a.tiles[t];
+        const TileEnt te = ld_ro(a.tiles + t);
         const uint32_t r = wave_uniform(te.r);
-        const RealInfo ri = a.info[r];
-        const RealCtl* cp = &a.ctl[r];
+        const RealInfo ri = ld_ro(a.info + r);
+        const cptr<RealCtl> cp = ro(a.ctl + r);
         if (wave_uniform(cp->done) || wave_uniform(cp->retry) || wave_uniform(cp->fallback)) continue;
         const uint32_t* X = wave_uniform(cp->cur) ? a.x1 : a.x0;
         const double prev = cp->tau;
