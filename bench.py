@@ -40,7 +40,8 @@ def algorithmic_bytes(S, topos, part):
     out plus the topology once), "leap_kernel" (phase_b_kernel of the
     edge-parallel pass: counts, the forward CSR -- each undirected edge once
     -- and the 64-bit balances it produces), "finish" (phase_b_finish_kernel:
-    counts and balances in, new counts out)."""
+    counts and balances in, new counts out), "step" (one whole step:
+    N (12 S + 10) + 2 topology)."""
     tot = 0
     for t in topos:
         N = int(t["N"])
@@ -54,6 +55,10 @@ def algorithmic_bytes(S, topos, part):
             tot += 4 * S * N + 4 * (N + 1) + 2 * E + (2 * E if wt else 0) + 8 * S * N
         elif part == "finish":
             tot += 4 * S * N + 8 * S * N + 4 * S * N
+        elif part == "step":
+            # the minimal two-pass step: counts read twice and written once,
+            # a 10-byte tau / flag per subvolume, the topology read twice
+            tot += N * (12 * S + 10) + 2 * (4 * (N + 1) + 4 * E + (4 * E if wt else 0))
         else:
             raise ValueError(part)
     return tot
@@ -295,6 +300,9 @@ def main():
     dom_ms, dom_bytes = cand[kname]
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     traffic = load_traffic(f"{args.workload}:{kname}")
+    # whole-step compulsory bytes x accepted steps / wall time
+    b_step = algorithmic_bytes(S, wl["topos"], "step")
+    b_step_gbs = b_step * res["accepted_steps"] / len(wl["topos"]) / res["seconds"] / 1e9
     out = {
         "metric": "subvolume-leaps/s (plain Cao tau), 1 MI355X",
         "value": value,
@@ -327,6 +335,8 @@ def main():
                                                "phase_b_redo_kernel": res["redo_ms"],
                                                "phase_b_finish_kernel": res["finish_ms"]},
                     "per_kernel_GBs": {k: b / (ms * 1e-3) / 1e9 for k, (ms, b) in cand.items() if ms > 0}},
+        "step_roofline": {"B_step": b_step, "achieved": b_step_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": b_step_gbs / HBM_PEAK_GBS},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic},

@@ -1,7 +1,9 @@
 """Checkpoint / resume through the C ABI: a context restored from a
 checkpoint continues bit for bit like the one that wrote it (counts, clock,
 step statistics), for a single grid with leap-backs and critical events and
-for an uneven batch; a checkpoint of another seed or epsilon is refused."""
+for an uneven batch, also when restored into a context that ran on; a
+checkpoint of another seed or epsilon is refused.  Also the per-kernel
+timing entry point."""
 import numpy as np
 import pytest
 
@@ -66,3 +68,32 @@ def test_restore_refuses_other_seed_or_eps():
     with pytest.raises(capi.PlisError):
         b.restore(blob[:100])
     b.close()
+
+
+def test_restore_into_a_context_that_ran_on():
+    """Restoring into the context that wrote the checkpoint, after it ran
+    further (its leap balance buffers hold the later steps' values), gives
+    the same continuation as the first time."""
+    net, topo, x0 = H.small_bruss()
+    a = capi.Context(net, [topo], [x0], eps=0.03, seed=5)
+    a.run(np.inf, step_limit=20)
+    blob = a.checkpoint()
+    a.run(np.inf, step_limit=45)
+    first, st1 = a.get_state(), a.stats()
+    a.run(np.inf, step_limit=70)
+    a.restore(blob)
+    a.run(np.inf, step_limit=45)
+    assert np.array_equal(a.get_state(), first)
+    assert all(a.stats()[k] == st1[k] for k in STAT_KEYS)
+    a.close()
+
+
+def test_profile_kernels_splits_the_leap_pass():
+    net, topo, x0 = H.small_bruss()
+    a = capi.Context(net, [topo], [x0], eps=0.03, seed=5)
+    a.arm(t_end=float("inf"))
+    p = a.profile_kernels(5)
+    assert len(p) == 8 and all(v >= 0.0 for v in p)
+    assert p[5] > 0.0 and p[7] > 0.0  # edge-parallel: draws and finish kernels ran
+    assert p[5] + p[6] + p[7] <= p[3] * 1.5 + 0.05
+    a.close()
