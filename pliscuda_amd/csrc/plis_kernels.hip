@@ -141,6 +141,16 @@ __device__ __forceinline__ double g_cao(int hor, int mult, uint32_t xs) {
     return 3.0;
 }
 
+// eps x / g(x): for g = 1 and g = 2 (first order; second order with one
+// molecule of the species) the quotient is formed without a divide -- x / 1
+// and x / 2 are exact, so the result is the same IEEE value.
+__device__ __forceinline__ double eps_x_over_g(double eps, int hor, int mult, uint32_t xs) {
+    const double ex = eps * (double)xs;
+    if (hor == 1) return ex;
+    if (hor == 2 && mult != 2) return ex * 0.5;
+    return ex / g_cao(hor, mult, xs);
+}
+
 // Post-leap check (Anderson 2008 form of Cao's leap condition): a reactant
 // species whose realized change exceeds theta * max(eps x / g, 1) (species
 // of highest order 0 are free).
@@ -148,7 +158,7 @@ __device__ __forceinline__ bool species_violates(int hor, int mult, uint32_t x, 
                                                  double theta) {
     if (hor == 0) return false;
     const double dx = __builtin_fabs((double)nx - (double)x);
-    const double bound = theta * fmax((eps * (double)x) / g_cao(hor, mult, x), 1.0);
+    const double bound = theta * fmax(eps_x_over_g(eps, hor, mult, x), 1.0);
     return dx > bound;
 }
 
@@ -381,8 +391,7 @@ __device__ __forceinline__ double tau_bound(const KernelArgs& a, const NetView& 
         if (s >= n.S) break;
         const int hor = n.hor[s];
         if (hor == 0) continue;
-        const double gg = g_cao(hor, n.mult[s], t.x[s]);
-        const double mm = fmax((eps * (double)t.x[s]) / gg, 1.0);
+        const double mm = fmax(eps_x_over_g(eps, hor, n.mult[s], t.x[s]), 1.0);
         const double t1 = mm / __builtin_fabs(t.mu[s]);
         const double t2 = (mm * mm) / t.sig[s];
         tp = fmin(tp, fmin(t1, t2));
