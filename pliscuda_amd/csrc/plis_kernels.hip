@@ -1209,10 +1209,10 @@ __device__ __forceinline__ StepView step_view(const RealInfo& ri, cptr<RealCtl> 
 }
 
 // Phase B of one tile.  EP: the draws of an edge-parallel tile into the
-// global balances (FULL = complete sampler, the redo pass); the tile's
+// global balances (REDO: the open draws, complete sampler); the tile's
 // firings go to its partial and phase_b_finish_kernel completes the leap.
 // Otherwise one lane walks one subvolume's out- and in-edges and finishes it.
-template <int MS, int MR, bool EP, bool FULL>
+template <int MS, int MR, bool EP, bool REDO>
 __device__ __forceinline__ void phase_b_tile(const KernelArgs& a, const NetView& n, uint32_t t,
                                              uint32_t (*s_x)[kTile], unsigned long long (*s_del)[kTile],
                                              uint32_t* s_rp, int32_t* s_c, uint32_t* s_q, uint32_t* s_qn) {
@@ -1229,7 +1229,7 @@ __device__ __forceinline__ void phase_b_tile(const KernelArgs& a, const NetView&
     unsigned long long fire = 0;
     uint32_t bad = 0, viol = 0;
     if constexpr (EP) {
-        const bool complete = leap_tile_fwd<MS, MR, FULL>(a, n, ri, X, te.v0, sv, s_x, s_del, s_rp, s_c, s_q, s_qn, fire);
+        const bool complete = leap_tile_fwd<MS, MR, REDO>(a, n, ri, X, te.v0, sv, s_x, s_del, s_rp, s_c, s_q, s_qn, fire);
         reduce_partB(fire, bad, viol);
         if (lane == 0) {
             if (!complete) {
@@ -1237,7 +1237,7 @@ __device__ __forceinline__ void phase_b_tile(const KernelArgs& a, const NetView&
                 if (slot < a.ntiles) a.redo_tiles[slot] = t;  // one entry per tile per pass
             }
             // the redo pass adds the firings of the open draws
-            a.partB_tile[t] = PartB{(FULL ? a.partB_tile[t].fire : 0ull) + fire, 0u, 0u};
+            a.partB_tile[t] = PartB{(REDO ? a.partB_tile[t].fire : 0ull) + fire, 0u, 0u};
         }
         return;
     } else {
