@@ -31,6 +31,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 PMC_TRAFFIC = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+PMC_VALU = os.path.join(ROOT, "profiles", "pmc_valu.json")
 
 
 def algorithmic_bytes(S, topos, part):
@@ -246,13 +247,30 @@ def copy_probe(nbytes=1 << 28, reps=5):
     print(json.dumps({"copy_probe_bytes": nbytes, "reps": reps}), flush=True)
 
 
-def load_traffic(kernel):
+def load_traffic(kernel, path=None):
     try:
-        with open(PMC_TRAFFIC) as f:
+        with open(path or PMC_TRAFFIC) as f:
             d = json.load(f)
         return d.get(kernel)
     except (OSError, ValueError):
         return None
+
+
+# Instruction-issue bound: a wave64 VALU instruction holds one SIMD for 4
+# cycles (16 lanes per cycle); MI355X has 256 CUs x 4 SIMDs at up to 2.4 GHz.
+VALU_SIMDS, VALU_CYCLES_PER_INST, CLOCK_HZ = 1024, 4, 2.4e9
+
+
+def valu_issue(workload, kname, ms):
+    """VALU instructions per launch (rocprofv3 SQ_INSTS_VALU, committed
+    under profiles/) against the launch's measured duration: the fraction
+    of the chip's VALU issue slots the kernel fills."""
+    n = load_traffic(f"{workload}:{kname}", PMC_VALU)
+    if not n or not ms:
+        return None
+    t_min = n * VALU_CYCLES_PER_INST / VALU_SIMDS / CLOCK_HZ
+    return {"kernel": kname, "valu_instructions": n, "min_us": t_min * 1e6, "measured_us": ms * 1e3,
+            "frac": t_min / (ms * 1e-3)}
 
 
 def main():
@@ -337,6 +355,7 @@ def main():
                     "per_kernel_GBs": {k: b / (ms * 1e-3) / 1e9 for k, (ms, b) in cand.items() if ms > 0}},
         "step_roofline": {"B_step": b_step, "achieved": b_step_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": b_step_gbs / HBM_PEAK_GBS},
+        "valu_issue": [v for v in (valu_issue(args.workload, k, ms) for k, (ms, _) in cand.items()) if v],
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic},
