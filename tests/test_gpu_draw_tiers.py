@@ -10,7 +10,8 @@ cases drive each branch on purpose -- draws queued and drained, queues that
 overflow on some tiles only, every tile redone -- predict on the host which
 tiles the leap pass must hand to the redo pass (the same draws, rebuilt from
 the oracle's Philox and squeeze restatement), check the GPU's count against
-that, and check the trajectories bit for bit against the oracle.
+that, and check the trajectories bit for bit against the oracle; one case
+runs the multi-compartment instantiations of the same kernels.
 """
 import numpy as np
 import pytest
@@ -173,3 +174,29 @@ def test_large_means_redo_every_tile():
     und, big = _predict_step0(net, topo, x0, 0.3)
     assert big.all()
     _first_steps_match(net, topo, x0, 0.3, 10, expect_redo=len(big))
+
+
+def test_large_means_with_compartments_redo_every_tile():
+    """Two compartments (per-subvolume rate and diffusion tables: the
+    multi-compartment instantiations of the leap, redo and finish kernels)
+    with means in the hundreds: every tile goes to the redo pass on the
+    first step, and the trajectory stays bit-identical to the oracle."""
+    net = W.network(["X", "Y"], [({}, {"X": 1}), ({"X": 1}, {}), ({"X": 1}, {"Y": 1}), ({"Y": 1}, {})],
+                    [[5000.0, 5.0, 0.5, 1.0], [3000.0, 2.0, 1.0, 0.5]], [[0.1, 0.2], [0.3, 0.05]],
+                    n_compartments=2)
+    topo = dict(W.cubic_grid(8, 8, 8, periodic=True))
+    topo["comp"] = (np.arange(topo["N"]) // 100 % 2).astype(np.int32)
+    x0 = W.poisson_state(2, topo["N"], [1000.0, 800.0], 9)
+    ctx = capi.Context(net, [topo], [x0], eps=0.3, seed=4)
+    ctx.run(np.inf, step_limit=1)
+    st = ctx.stats()
+    assert st["leapbacks"] == 0 and ctx.redo_tiles() == topo["N"] // 64
+    ctx.run(np.inf, step_limit=8)
+    m = ffi.Model(net, topo)
+    xo = x0.copy()
+    so = ffi.run(m, xo, 0.3, 10, 4, np.inf, step_limit=8)
+    st = ctx.stats()
+    assert np.array_equal(ctx.get_state(), xo), "state differs"
+    for f in ("steps", "leapbacks", "firings", "subvolume_leaps"):
+        assert st[f] == getattr(so, f), f
+    ctx.close()
