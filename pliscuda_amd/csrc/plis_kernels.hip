@@ -419,9 +419,17 @@ __device__ __forceinline__ void tau_subvolume(const KernelArgs& a, const NetView
     double du[MS];
 #pragma unroll
     for (int s = 0; s < MS; ++s) du[s] = n.d[s < n.S ? s : 0];
+    // the next in-edge's id and weight are fetched while this one's counts
+    // load (one dependent round trip per in-edge instead of two)
+    uint32_t u_next = ib < ie ? icol[ib] : 0u;
+    float w_next = (ib < ie && weighted) ? iwp[ib] : 1.0f;
     for (uint32_t q = ib; q < ie; ++q) {
-        const uint32_t u = icol[q];
-        const double wq = weighted ? (double)iwp[q] : 1.0;
+        const uint32_t u = u_next;
+        const double wq = (double)w_next;
+        if (q + 1u < ie) {
+            u_next = icol[q + 1u];
+            w_next = weighted ? iwp[q + 1u] : 1.0f;
+        }
         if (n.C > 1) {
             const int cu = a.comp[ri.comp_off + u];
 #pragma unroll
