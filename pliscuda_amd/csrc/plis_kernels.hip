@@ -419,28 +419,38 @@ __device__ __forceinline__ void tau_subvolume(const KernelArgs& a, const NetView
     double du[MS];
 #pragma unroll
     for (int s = 0; s < MS; ++s) du[s] = n.d[s < n.S ? s : 0];
-    // the next in-edge's id and weight are fetched while this one's counts
-    // load (one dependent round trip per in-edge instead of two)
-    uint32_t u_next = ib < ie ? icol[ib] : 0u;
-    float w_next = (ib < ie && weighted) ? iwp[ib] : 1.0f;
+    // software pipeline: in-edge q's counts were requested in iteration q-1,
+    // in-edge q+1's id and weight in iteration q-1 too, so each iteration
+    // waits on nothing issued in it
+    uint32_t u_cur = ib < ie ? icol[ib] : v;
+    float w_cur = (ib < ie && weighted) ? iwp[ib] : 1.0f;
+    uint32_t u_next = ib + 1u < ie ? icol[ib + 1u] : v;
+    float w_next = (ib + 1u < ie && weighted) ? iwp[ib + 1u] : 1.0f;
+    uint32_t xu[MS];
+#pragma unroll
+    for (int s = 0; s < MS; ++s) xu[s] = X[ri.x_off + (uint64_t)(s < n.S ? s : n.S - 1) * N + u_cur];
     for (uint32_t q = ib; q < ie; ++q) {
-        const uint32_t u = u_next;
-        const double wq = (double)w_next;
+        const uint32_t u = u_cur;
+        const double wq = (double)w_cur;
+        uint32_t xc[MS];
+#pragma unroll
+        for (int s = 0; s < MS; ++s) xc[s] = xu[s];
+        u_cur = u_next;
+        w_cur = w_next;
         if (q + 1u < ie) {
-            u_next = icol[q + 1u];
-            w_next = weighted ? iwp[q + 1u] : 1.0f;
+#pragma unroll
+            for (int s = 0; s < MS; ++s) xu[s] = X[ri.x_off + (uint64_t)(s < n.S ? s : n.S - 1) * N + u_cur];
+            if (q + 2u < ie) {
+                u_next = icol[q + 2u];
+                w_next = weighted ? iwp[q + 2u] : 1.0f;
+            }
         }
         if (n.C > 1) {
             const int cu = a.comp[ri.comp_off + u];
 #pragma unroll
             for (int s = 0; s < MS; ++s) du[s] = n.d[cu * n.S + (s < n.S ? s : 0)];
         }
-        // every count of the neighbour requested at once (clamped species
-        // index: the loads stay in bounds and unconditional)
-        uint32_t xu[MS];
-#pragma unroll
-        for (int s = 0; s < MS; ++s) xu[s] = X[ri.x_off + (uint64_t)(s < n.S ? s : n.S - 1) * N + u];
-        tau_in_edge<MS>(n.S, du, xu, wq, a.n_critical, t);
+        tau_in_edge<MS>(n.S, du, xc, wq, a.n_critical, t);
     }
     tp_out = tau_bound<MS>(a, n, t);
     a0c_out = t.a0c;
