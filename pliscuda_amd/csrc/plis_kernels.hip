@@ -879,11 +879,37 @@ __device__ __forceinline__ void walk_fwd_draws(const KernelArgs& a, const NetVie
     // fast-path firings of this lane: each settled draw is <= 15 and a tile
     // has < 2^27 entries, so 32 bits hold them
     uint32_t fire_fast = 0u;
-    for (uint32_t e = eb + lane; e < ee; e += (uint32_t)kTile) {
+    // software pipeline: an entry's neighbour counts are requested one
+    // iteration ahead, its neighbour id and weight two iterations ahead
+    // (counts of a neighbour inside the tile are read from global memory
+    // too: the staged copy holds the same values)
+    auto counts = [&](uint32_t u, uint32_t (&xo)[MS]) {
+#pragma unroll
+        for (int s = 0; s < MS; ++s) xo[s] = X[ri.x_off + (uint64_t)(s < n.S ? s : n.S - 1) * N + u];
+    };
+    const uint32_t e0 = eb + lane;
+    uint32_t u_cur = e0 < ee ? colp[e0] : v0, u_next = e0 + (uint32_t)kTile < ee ? colp[e0 + kTile] : v0;
+    float w_cur = (e0 < ee && weighted) ? wp[e0] : 1.0f;
+    float w_next = (e0 + (uint32_t)kTile < ee && weighted) ? wp[e0 + kTile] : 1.0f;
+    uint32_t xn[MS];
+    if (e0 < ee) counts(u_cur, xn);
+    for (uint32_t e = e0; e < ee; e += (uint32_t)kTile) {
         const uint32_t i = edge_owner(s_rp, e);
         const uint32_t vo = v0 + i;
-        const uint32_t u = colp[e];  // u > vo: vo is the edge's lower end
-        const float wf = weighted ? wp[e] : 1.0f;
+        const uint32_t u = u_cur;  // u > vo: vo is the edge's lower end
+        const float wf = w_cur;
+        uint32_t xc[MS];
+#pragma unroll
+        for (int s = 0; s < MS; ++s) xc[s] = xn[s];
+        u_cur = u_next;
+        w_cur = w_next;
+        if (e + (uint32_t)kTile < ee) {
+            counts(u_cur, xn);
+            if (e + 2u * (uint32_t)kTile < ee) {
+                u_next = colp[e + 2u * kTile];
+                w_next = weighted ? wp[e + 2u * kTile] : 1.0f;
+            }
+        }
         const bool local = u - v0 < (uint32_t)kTile;
         const int co = n.C > 1 ? s_c[i] : 0;
         const int cu = n.C > 1 ? (local ? s_c[u - v0] : a.comp[ri.comp_off + u]) : 0;
@@ -895,8 +921,8 @@ __device__ __forceinline__ void walk_fwd_draws(const KernelArgs& a, const NetVie
             const int s1 = s0 + 1 < MS ? s0 + 1 : s0;
             const uint32_t xo0 = s_x[s0][i], xo1 = two ? s_x[s1][i] : 0u;
             const double do0 = n.d[co * n.S + s0], do1 = two ? n.d[co * n.S + s1] : 0.0;
-            const uint32_t xu0 = local ? s_x[s0][u - v0] : X[ri.x_off + (uint64_t)s0 * N + u];
-            const uint32_t xu1 = two ? (local ? s_x[s1][u - v0] : X[ri.x_off + (uint64_t)s1 * N + u]) : 0u;
+            const uint32_t xu0 = xc[s0];
+            const uint32_t xu1 = two ? xc[s1] : 0u;
             const double du0 = n.d[cu * n.S + s0], du1 = two ? n.d[cu * n.S + s1] : 0.0;
             const uint32_t blk = (uint32_t)s0 >> 1;
             const uint32_t cb = c3hop | blk;
