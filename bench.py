@@ -78,16 +78,34 @@ def build_workload(name):
     raise SystemExit(f"unknown workload {name}")
 
 
+def tuned_shape(wl, seed=2024):
+    """(waves per block, blocks per CU) the autotuner picks for a workload,
+    found on a throwaway context."""
+    import torch
+
+    from pliscuda_amd import capi
+    ctx = capi.Context(wl["net"], wl["topos"], wl["x0"], eps=wl["eps"], nc=wl["nc"], seed=seed)
+    ctx.arm(t_end=float("inf"))
+    ctx.autotune(3)
+    sh = ctx.launch_shape()
+    ctx.close()
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    return sh["waves_per_block"], -(-sh["blocks"] // cus)
+
+
 def gpu_measure(wl, steps, warmup, seed=2024, autotune=True, shape=None):
     import torch
 
     from pliscuda_amd import capi
     wpb, bpc = shape if shape else (0, 0)
+    if autotune and not shape:
+        # the autotuner times real iterations, which advance the trajectory:
+        # tune on a context of its own so that the timed run starts from the
+        # initial state (as the learned leg does)
+        wpb, bpc = tuned_shape(wl, seed)
     ctx = capi.Context(wl["net"], wl["topos"], wl["x0"], eps=wl["eps"], nc=wl["nc"], seed=seed,
                        waves_per_block=wpb, blocks_per_cu=bpc)
     ctx.arm(t_end=float("inf"))
-    if autotune and not shape:
-        ctx.autotune(3)
     shape = ctx.launch_shape()
     ctx.iterate(warmup)
     ctx.sync()
@@ -112,14 +130,14 @@ def gpu_measure(wl, steps, warmup, seed=2024, autotune=True, shape=None):
     prof += ctx.profile_kernels(n_prof)[5:]
     ctx.close()
     return {"seconds": dt, "accepted_steps": acc, "subvolume_leaps": leaps, "firings": fir,
-            "leapbacks": lbs, "subvolume_simtime": simt, "shape": shape,
+            "leapbacks": lbs, "subvolume_simtime": simt, "shape": shape, "tuned": (wpb, bpc),
             "tau_ms": prof[1] / n_prof, "decide_ms": prof[2] / n_prof, "leap_ms": prof[3] / n_prof,
             "commit_ms": prof[4] / n_prof, "iter_ms_events": prof[0] / n_prof,
             "leap_kernel_ms": prof[5] / n_prof, "redo_ms": prof[6] / n_prof, "finish_ms": prof[7] / n_prof}
 
 
 def learned_measure(wl, steps, warmup, train_steps, batch, seed=2024, kappa=1.0, snapshots=16,
-                    per_snapshot=16384):
+                    per_snapshot=16384, shape=None):
     """Records from plain steps of the same workload, GPU training, then the
     learned step timed like the plain one."""
     import torch
@@ -138,7 +156,9 @@ def learned_measure(wl, steps, warmup, train_steps, batch, seed=2024, kappa=1.0,
     t_train = time.perf_counter() - t0
     w = rec.mlp_get()
     rec.close()
-    ctx = capi.Context(wl["net"], wl["topos"], wl["x0"], eps=wl["eps"], nc=wl["nc"], seed=seed)
+    wpb, bpc = shape if shape else (0, 0)
+    ctx = capi.Context(wl["net"], wl["topos"], wl["x0"], eps=wl["eps"], nc=wl["nc"], seed=seed,
+                       waves_per_block=wpb, blocks_per_cu=bpc)
     ctx.mlp_set(w, norm)
     ctx.set_tau_mode(True, kappa=kappa, theta=3.0, viol_rate_max=vmax)
     ctx.arm(t_end=float("inf"))
@@ -362,7 +382,7 @@ def main():
     }
     if not args.no_learned and args.workload == "brusselator":
         lr_ = learned_measure(wl, args.steps, args.warmup, args.train_steps, args.train_batch,
-                              kappa=args.kappa)
+                              kappa=args.kappa, shape=res["tuned"])
         lv = lr_["subvolume_leaps"] / lr_["seconds"]
         simt_plain = res["subvolume_simtime"] / res["seconds"]
         simt_learned = lr_["subvolume_simtime"] / lr_["seconds"]
@@ -387,7 +407,7 @@ def main():
         extra["abc_reflecting_8^3_x64_to_t10"] = abc_to_t_end()
         wb = build_workload("bruss_batch")
         rp = gpu_measure(wb, 100, 10)
-        rl = learned_measure(wb, 100, 10, 5000, 65536, snapshots=8, per_snapshot=512)
+        rl = learned_measure(wb, 100, 10, 5000, 65536, snapshots=8, per_snapshot=512, shape=rp["tuned"])
         vp, vl = rp["subvolume_leaps"] / rp["seconds"], rl["subvolume_leaps"] / rl["seconds"]
         sp, sl = rp["subvolume_simtime"] / rp["seconds"], rl["subvolume_simtime"] / rl["seconds"]
         extra[wb["name"] + "_learned"] = {
