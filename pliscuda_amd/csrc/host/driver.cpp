@@ -8,6 +8,7 @@
 // pause never shortens a step, so snapshots do not perturb the trajectory).
 #include "driver.hpp"
 
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <ctime>
@@ -115,25 +116,39 @@ int run_simulation(Topology t, State s, Reactions r, float* h_rrc, float* h_drc,
     const double t_end = (double)endTime;
     const uint64_t no_limit = std::numeric_limits<uint64_t>::max();
     std::vector<uint32_t> x((size_t)S * N);
-    double last_log = 0.0;
+    // per realization: the clock of its last snapshot (realizations step
+    // with their own tau, so they reach their snapshot times at different
+    // global iterations)
+    std::vector<double> last_log(nreal, 0.0);
     if (!opts.restore.empty()) {
-        plis_stats st;
-        plis_get_stats(ctx, 0, &st);
-        last_log = st.t;  // snapshots continue from the restored clock
+        for (int q = 0; q < nreal; ++q) {
+            plis_stats st;
+            plis_get_stats(ctx, q, &st);
+            last_log[q] = st.t;  // snapshots continue from the restored clock
+        }
     }
     bool ok = true;
     if (log_freq > 0) {
         for (;;) {
-            rc = plis_run_until(ctx, t_end, last_log + (double)log_freq, no_limit);
-            if (rc != PLIS_OK) break;
-            plis_stats st;
-            plis_get_stats(ctx, 0, &st);
-            if (st.status != 0 || st.t >= t_end) break;
-            last_log = st.t;
+            // pause at the earliest snapshot time still due; a realization
+            // already past it does not step in this call
+            double t_pause = std::numeric_limits<double>::infinity();
             for (int q = 0; q < nreal; ++q) {
-                plis_get_state(ctx, q, x.data());
                 plis_stats sq;
                 plis_get_stats(ctx, q, &sq);
+                if (sq.status == 0 && sq.t < t_end) t_pause = std::min(t_pause, last_log[q] + (double)log_freq);
+            }
+            if (t_pause == std::numeric_limits<double>::infinity()) break;  // every realization finished
+            rc = plis_run_until(ctx, t_end, t_pause, no_limit);
+            if (rc != PLIS_OK) break;
+            for (int q = 0; q < nreal; ++q) {
+                plis_stats sq;
+                plis_get_stats(ctx, q, &sq);
+                // a realization paused on crossing its own snapshot time
+                // (finished ones get their final snapshot below)
+                if (sq.status != 0 || sq.t >= t_end || sq.t - last_log[q] < (double)log_freq) continue;
+                last_log[q] = sq.t;
+                plis_get_state(ctx, q, x.data());
                 write_snapshot(tstamp, q, nreal, x, S, N, sq.t);
             }
         }

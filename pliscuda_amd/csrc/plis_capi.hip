@@ -22,6 +22,8 @@ using namespace plis;
 
 static thread_local std::string g_create_err;
 
+constexpr int kGraphSizes = 7;
+
 struct plis_ctx {
     hipStream_t stream = nullptr;
     int device = 0;
@@ -32,6 +34,7 @@ struct plis_ctx {
     int32_t S = 0, R = 0, C = 1;
     uint32_t nreal = 0, ntiles = 0;
     uint64_t total_counts = 0, total_sub = 0;
+    uint64_t model_hash = 0;  // network tables, every realization's CSR and compartments
     std::vector<RealInfo> info;
     std::vector<RealCtl> ctl_host;
     // device buffers
@@ -60,8 +63,8 @@ struct plis_ctx {
     int use_graph = 1;
     bool edge_parallel = false;  // every realization symmetric, tiles < 2^27 forward entries
     hipEvent_t* split_ev = nullptr;  // profiling: events between the leap pass's kernels
-    hipGraphExec_t graph = nullptr;
-    uint32_t graph_iters = 0;
+    // captured step sequences of 1, 2, 4, ..., 64 iterations (index log2 n)
+    hipGraphExec_t graphs[kGraphSizes] = {};
     std::vector<void*> allocs;
     // post-leap check and learned tau
     double theta = 3.0, viol_rate_max = 1.0, kappa = 1.0;
@@ -102,6 +105,21 @@ static int upload(plis_ctx* c, T** p, const std::vector<T>& v) {
 static int fail_create(const std::string& m) {
     g_create_err = m;
     return PLIS_ERR_ARG;
+}
+
+// 64-bit FNV-1a over the bytes of the model a checkpoint belongs to.
+static void hash_bytes(uint64_t& h, const void* p, size_t n) {
+    const unsigned char* b = static_cast<const unsigned char*>(p);
+    for (size_t i = 0; i < n; ++i) {
+        h ^= b[i];
+        h *= 0x100000001B3ull;
+    }
+}
+template <typename T>
+static void hash_vec(uint64_t& h, const std::vector<T>& v) {
+    const uint64_t n = v.size();
+    hash_bytes(h, &n, sizeof n);
+    if (!v.empty()) hash_bytes(h, v.data(), v.size() * sizeof(T));
 }
 
 // ---------------------------------------------------------------------------
@@ -358,17 +376,23 @@ static void launch_step(plis_ctx* c, const KernelArgs& a) {
     for (int k = 0; k < 4; ++k) launch_kernel(c, k, a);
 }
 
+// The captured graphs hold the kernel arguments and launch shape by value:
+// every change of either drops them.
+static void drop_graphs(plis_ctx* c) {
+    for (auto& g : c->graphs)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+}
+
 static void set_shape(plis_ctx* c, uint32_t wpb, uint32_t bpc) {
     c->wpb = std::max(1u, std::min(4u, wpb));
     const uint32_t need = (c->ntiles + c->wpb - 1) / c->wpb;
     uint32_t blocks = std::min<uint32_t>((uint32_t)c->cus * std::max(1u, bpc), need);
     blocks = std::max(8u, (blocks + 7u) & ~7u);
     c->blocks = blocks;
-    if (c->graph) {
-        (void)hipGraphExecDestroy(c->graph);
-        c->graph = nullptr;
-        c->graph_iters = 0;
-    }
+    drop_graphs(c);
 }
 
 static int launch_iters(plis_ctx* c, uint32_t n) {
@@ -378,23 +402,34 @@ static int launch_iters(plis_ctx* c, uint32_t n) {
     return 0;
 }
 
+// n iterations as replays of captured graphs of 64, 32, ..., 1 iterations
+// (binary decomposition of n).  The first replay captures every size at once,
+// so a later call of another length does not capture inside a timed region.
 static int graph_iters(plis_ctx* c, uint32_t n) {
     if (!c->use_graph) return launch_iters(c, n);
-    if (c->graph_iters != n) {
-        if (c->graph) {
-            (void)hipGraphExecDestroy(c->graph);
-            c->graph = nullptr;
-        }
-        hipGraph_t g;
-        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    if (!c->graphs[0]) {
         const KernelArgs a = make_args(c);
-        for (uint32_t i = 0; i < n; ++i) launch_step(c, a);
-        HIPCHK(c, hipStreamEndCapture(c->stream, &g));
-        HIPCHK(c, hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0));
-        HIPCHK(c, hipGraphDestroy(g));
-        c->graph_iters = n;
+        for (int k = 0; k < kGraphSizes; ++k) {
+            hipGraph_t g;
+            HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            for (uint32_t i = 0; i < (1u << k); ++i) launch_step(c, a);
+            HIPCHK(c, hipStreamEndCapture(c->stream, &g));
+            const hipError_t e = hipGraphInstantiate(&c->graphs[k], g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) {
+                c->graphs[k] = nullptr;
+                drop_graphs(c);
+                c->err = std::string("hipGraphInstantiate: ") + hipGetErrorString(e);
+                return PLIS_ERR_HIP;
+            }
+        }
     }
-    HIPCHK(c, hipGraphLaunch(c->graph, c->stream));
+    while (n > 0) {
+        int k = kGraphSizes - 1;
+        while ((1u << k) > n) --k;
+        HIPCHK(c, hipGraphLaunch(c->graphs[k], c->stream));
+        n -= 1u << k;
+    }
     return 0;
 }
 
@@ -422,7 +457,7 @@ extern "C" void plis_destroy(plis_ctx* c) {
     if (!c) return;
     // teardown: release everything even if an earlier call failed
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    if (c->graph) (void)hipGraphExecDestroy(c->graph);
+    drop_graphs(c);
     for (void* p : c->allocs) (void)hipFree(p);
     if (c->h_ndone) (void)hipHostFree(c->h_ndone);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -453,6 +488,9 @@ extern "C" int plis_create(const plis_network* net, int32_t n_real, const plis_t
     c->S = nh.S;
     c->R = nh.R;
     c->C = nh.C;
+    c->model_hash = 0xCBF29CE484222325ull;
+    hash_vec(c->model_hash, I);
+    hash_vec(c->model_hash, D);
     c->variant = 4;
     for (int v = 3; v >= 0; --v)
         if (nh.S <= kVariantS[v] && nh.R <= kVariantR[v]) c->variant = v;
@@ -486,6 +524,9 @@ extern "C" int plis_create(const plis_network* net, int32_t n_real, const plis_t
         bool weighted = false;
         for (float wv : g.w) weighted = weighted || wv != 1.0f;
         const bool sym = is_symmetric(g, N);
+        hash_vec(c->model_hash, g.rp);
+        hash_vec(c->model_hash, g.col);
+        hash_vec(c->model_hash, g.w);
         RealInfo ri{};
         ri.x_off = xoff;
         ri.e_off = eoff;
@@ -539,6 +580,7 @@ extern "C" int plis_create(const plis_network* net, int32_t n_real, const plis_t
                 if (cv < 0 || cv >= nh.C) { delete c; return fail_create("compartment id out of range"); }
                 hcomp.push_back(cv);
             }
+            hash_bytes(c->model_hash, hcomp.data() + coff, (size_t)N * sizeof(int32_t));
             coff += N;
         }
         hx.insert(hx.end(), x0[r], x0[r] + (size_t)S * N);
@@ -697,11 +739,7 @@ extern "C" int plis_run(plis_ctx* c, double t_end, uint64_t step_limit) {
 
 extern "C" int plis_iterate(plis_ctx* c, uint32_t n) {
     if (!c) return PLIS_ERR_ARG;
-    while (n > 0) {
-        const uint32_t k = std::min<uint32_t>(n, 64u);
-        if (int rc = (k == 64u ? graph_iters(c, k) : launch_iters(c, k))) return rc;
-        n -= k;
-    }
+    if (int rc = graph_iters(c, n)) return rc;
     return PLIS_OK;
 }
 
@@ -795,6 +833,7 @@ struct CkptHeader {
     double eps;
     int32_t nc;
     uint32_t ctl_bytes;
+    uint64_t model_hash;  // reaction tables, constants, every CSR and compartment table
 };
 constexpr char kCkptMagic[8] = {'P', 'L', 'I', 'S', 'C', 'K', 'P', 'T'};
 
@@ -819,7 +858,7 @@ extern "C" int plis_checkpoint(plis_ctx* c, void* buf, uint64_t bytes) {
     char* p = static_cast<char*>(buf);
     CkptHeader h{};
     std::memcpy(h.magic, kCkptMagic, sizeof h.magic);
-    h.version = 1u;
+    h.version = 2u;
     h.nreal = c->nreal;
     h.S = c->S;
     h.R = c->R;
@@ -828,6 +867,7 @@ extern "C" int plis_checkpoint(plis_ctx* c, void* buf, uint64_t bytes) {
     h.eps = c->eps;
     h.nc = c->nc;
     h.ctl_bytes = (uint32_t)sizeof(RealCtl);
+    h.model_hash = c->model_hash;
     std::memcpy(p, &h, sizeof h);
     p += sizeof h;
     for (uint32_t r = 0; r < c->nreal; ++r) std::memcpy(p + 4ull * r, &c->info[r].N, 4);
@@ -854,7 +894,7 @@ extern "C" int plis_restore(plis_ctx* c, const void* buf, uint64_t bytes) {
         return PLIS_ERR_ARG;
     }
     std::memcpy(&h, p, sizeof h);
-    if (std::memcmp(h.magic, kCkptMagic, sizeof h.magic) != 0 || h.version != 1u ||
+    if (std::memcmp(h.magic, kCkptMagic, sizeof h.magic) != 0 || h.version != 2u ||
         h.ctl_bytes != sizeof(RealCtl)) {
         c->err = "plis_restore: not a checkpoint of this library version";
         return PLIS_ERR_ARG;
@@ -862,6 +902,10 @@ extern "C" int plis_restore(plis_ctx* c, const void* buf, uint64_t bytes) {
     if (h.nreal != c->nreal || h.S != c->S || h.R != c->R || h.total_counts != c->total_counts ||
         h.seed != c->seed || h.eps != c->eps || h.nc != c->nc || bytes < ckpt_bytes(c)) {
         c->err = "plis_restore: checkpoint of a different model, size, seed, epsilon or n_c";
+        return PLIS_ERR_ARG;
+    }
+    if (h.model_hash != c->model_hash) {
+        c->err = "plis_restore: checkpoint of a different network, topology, edge weights or compartments";
         return PLIS_ERR_ARG;
     }
     p += sizeof h;
@@ -1198,11 +1242,7 @@ extern "C" int plis_set_tau_mode(plis_ctx* c, int32_t learned, double kappa, dou
     c->kappa = kappa;
     c->theta = theta;
     c->viol_rate_max = viol_rate_max;
-    if (c->graph) {
-        (void)hipGraphExecDestroy(c->graph);
-        c->graph = nullptr;
-        c->graph_iters = 0;
-    }
+    drop_graphs(c);
     if (int rc = pull_ctl(c)) return rc;
     for (auto& q : c->ctl_host) q.fallback = 0u;
     HIPCHK(c, hipMemcpyAsync(c->d_ctl, c->ctl_host.data(), c->ctl_host.size() * sizeof(RealCtl),

@@ -70,6 +70,34 @@ def test_restore_refuses_other_seed_or_eps():
     b.close()
 
 
+def test_restore_refuses_a_model_of_the_same_size():
+    """A checkpoint carries a hash of the network, every topology, the edge
+    weights and the compartments: one changed rate constant, one rewired edge
+    or one changed weight is refused, not restored silently."""
+    net, topo, x0 = H.small_bruss()
+    a = capi.Context(net, [topo], [x0], eps=0.03, seed=5)
+    a.run(np.inf, step_limit=3)
+    blob = a.checkpoint()
+    a.close()
+    net2 = dict(net)
+    net2["k"] = np.array(net["k"], dtype=np.float64).copy()
+    net2["k"][3] *= 1.5
+    topo_w = dict(topo)
+    topo_w["w"] = np.array(topo["w"], dtype=np.float32).copy()
+    # edge 0 (0 -> u) and its reverse u -> 0: the graph stays symmetric
+    u = int(topo["col"][0])
+    rev = int(topo["row_ptr"][u]) + int(np.searchsorted(topo["col"][topo["row_ptr"][u]:topo["row_ptr"][u + 1]], 0))
+    topo_w["w"][[0, rev]] = 1.25
+    for n_, t_ in ((net2, topo), (net, topo_w)):
+        b = capi.Context(n_, [t_], [x0], eps=0.03, seed=5)
+        with pytest.raises(capi.PlisError, match="different network"):
+            b.restore(blob)
+        b.close()
+    b = capi.Context(net, [topo], [x0], eps=0.03, seed=5)
+    b.restore(blob)  # the same model restores
+    b.close()
+
+
 def test_restore_into_a_context_that_ran_on():
     """Restoring into the context that wrote the checkpoint, after it ran
     further (its leap balance buffers hold the later steps' values), gives

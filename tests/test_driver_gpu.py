@@ -145,3 +145,34 @@ def test_checkpoint_restore_continues_the_run(tmp_path):
     (b / "conf.txt").write_text(conf_b + f"\nepsilon=0.05\nnc=10\nseed={SEED + 1}\nrestore={a / 'ck.bin'}\n")
     r = subprocess.run([DRIVER, "conf.txt"], cwd=b, capture_output=True, text=True, timeout=110)
     assert r.returncode != 0 and "cannot restore" in r.stderr
+
+
+def test_realizations_with_log_freq_all_reach_end_time(tmp_path):
+    """realizations=2 with logFreq: realizations step with their own tau, so
+    they cross their snapshot times at different iterations; every one of
+    them must still run to endTime, get its own periodic snapshots and end
+    where the oracle ends it."""
+    src = os.path.join(CASES, "react-9")
+    for f in os.listdir(src):
+        if f.endswith(".txt"):
+            shutil.copyfile(os.path.join(src, f), tmp_path / f)
+    conf = (tmp_path / "conf.txt").read_text().replace("logFreq=0", "logFreq=1")
+    (tmp_path / "conf.txt").write_text(conf.rstrip("\n") + f"\nepsilon=0.05\nnc=10\nseed={SEED}\nrealizations=2\n")
+    print("driver react-9 x2 realizations with logFreq=1", flush=True)
+    r = subprocess.run([DRIVER, "conf.txt"], cwd=tmp_path, capture_output=True, text=True, timeout=110)
+    assert r.returncode == 0, r.stdout + r.stderr
+    c = inputs.read_conf(os.path.join(src, "conf.txt"))
+    model = ffi.Model(c["net"], c["topo"])
+    for q in range(2):
+        snaps = glob.glob(str(tmp_path / f"sim*_r{q}_*.dat"))
+        times = sorted(float(re.search(r"_([0-9.]+)\.dat$", p).group(1)) for p in snaps)
+        assert len(times) >= 6, (q, times)  # >= one per time unit, plus the final
+        assert times[-1] == pytest.approx(c["t_end"], abs=1e-6), (q, times)
+        assert all(b - a >= 1.0 - 1e-5 for a, b in zip(times[:-2], times[1:-1])), (q, times)
+        final = max(snaps, key=lambda p: float(re.search(r"_([0-9.]+)\.dat$", p).group(1)))
+        lines = open(final).read().split("\n")
+        N = int(lines[0].split("=")[1])
+        x = np.array([[int(v) for v in lines[4 + i].split()] for i in range(N)], dtype=np.int64).T.ravel()
+        xo = c["x0"].copy()
+        ffi.run(model, xo, 0.05, 10, SEED, c["t_end"], realization=q)
+        assert np.array_equal(x, xo.astype(np.int64)), q

@@ -1,6 +1,6 @@
 """Statistical correctness of the tau-leap step (oracle on the CPU; the GPU
 is bit-identical to it, and tests/test_gpu_statistics.py repeats the main
-comparison through the HIP path).
+comparisons through the HIP path and compares the learned mode with it).
 
 * closed forms: birth-death stationary Poisson law, first-order decay
   (binomial), mean of pure diffusion e^{tQ} x0;
