@@ -77,6 +77,9 @@ def _declare(L):
     L.orc_sq_hop_check.restype = None
     L.orc_tau_all.argtypes = [P(Network), P(Topology), P(Params), P(C.c_uint32), C.c_uint32,
                               C.c_uint32, P(C.c_double), P(C.c_double), P(C.c_double)]
+    L.orc_critical_choice.argtypes = [P(Network), P(Topology), P(Params), P(C.c_uint32), C.c_uint32,
+                                      C.c_uint32, C.c_uint32, P(C.c_uint32), P(C.c_int32),
+                                      P(C.c_int32), P(C.c_uint32)]
     L.orc_run.argtypes = [P(Network), P(Topology), P(Params), C.c_uint32, P(C.c_uint32),
                           C.c_double, C.c_double, C.c_uint64, P(Stats)]
     L.orc_run_batch.argtypes = [P(Network), C.c_int32, P(Topology), P(Params), P(C.c_uint32),
@@ -215,6 +218,23 @@ def tau_all(model, x, eps, nc, seed, realization=0, step=0):
                            _ptr(a0c, C.c_double), _ptr(t2, C.c_double))
     assert rc == 0
     return tp, a0c, t2
+
+
+def critical_choice(model, x, eps, nc, seed, v, steps, realization=0):
+    """The oracle's critical-event choice at subvolume v for every step index
+    in steps: arrays (kind, idx, dst)."""
+    x = np.ascontiguousarray(x, dtype=np.uint32)
+    steps = np.ascontiguousarray(steps, dtype=np.uint32)
+    n = steps.size
+    kind = np.empty(n, np.int32)
+    idx = np.empty(n, np.int32)
+    dst = np.empty(n, np.uint32)
+    p = Params(eps, nc, seed)
+    rc = lib().orc_critical_choice(C.byref(model.net), C.byref(model.topo), C.byref(p),
+                                   _ptr(x, C.c_uint32), realization, v, n, _ptr(steps, C.c_uint32),
+                                   _ptr(kind, C.c_int32), _ptr(idx, C.c_int32), _ptr(dst, C.c_uint32))
+    assert rc == 0
+    return kind, idx, dst
 
 
 def run(model, x, eps, nc, seed, t_end, step_limit=2**63, realization=0, t_pause=float("inf"),

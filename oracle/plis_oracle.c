@@ -675,6 +675,30 @@ static void choose_event(const model_t *m, const csr_t *g, const orc_params *p, 
     }
 }
 
+/* The critical event the step loop would choose at subvolume v of state x
+ * for each of nsteps step indices (test access to choose_event): kind 1 =
+ * reaction idx, kind 2 = hop of species idx to dst, 0 = none. */
+int orc_critical_choice(const orc_network *net, const orc_topology *topo, const orc_params *p,
+                        const uint32_t *x, uint32_t realization, uint32_t v, uint32_t nsteps,
+                        const uint32_t *steps, int32_t *kind, int32_t *idx, uint32_t *dst)
+{
+    model_t m; csr_t g;
+    if (model_build(net, &m)) { model_free(&m); return -1; }
+    if (csr_build(topo, &g) || v >= g.N) { model_free(&m); csr_free(&g); return -1; }
+    for (uint32_t i = 0; i < nsteps; i++) {
+        ctl_t st;
+        memset(&st, 0, sizeof st);
+        st.vstar = v;
+        st.step = steps[i];
+        choose_event(&m, &g, p, x, realization, &st);
+        kind[i] = st.ev_kind;
+        idx[i] = st.ev_idx;
+        dst[i] = st.ev_dst;
+    }
+    model_free(&m); csr_free(&g);
+    return 0;
+}
+
 static void decide(ctl_t *st)
 {
     double tau = fmin(st->tau_p, st->tau2);

@@ -87,6 +87,12 @@ int orc_tau_all(const orc_network *net, const orc_topology *topo, const orc_para
                 const uint32_t *x, uint32_t realization, uint32_t step,
                 double *tau_prime, double *a0c, double *tau2);
 
+/* The critical event chosen at subvolume v for each of nsteps step indices
+ * (kind 1 = reaction idx, 2 = hop of species idx to dst, 0 = none). */
+int orc_critical_choice(const orc_network *net, const orc_topology *topo, const orc_params *p,
+                        const uint32_t *x, uint32_t realization, uint32_t v, uint32_t nsteps,
+                        const uint32_t *steps, int32_t *kind, int32_t *idx, uint32_t *dst);
+
 /* ---- the plain step loop ---- */
 /* Advance one realization until t >= t_end, steps >= step_limit, or
  * t >= t_pause (soft stop, no clamp).  x is SoA [s*N+v], in/out.
