@@ -160,7 +160,7 @@ def learned_measure(wl, steps, warmup, train_steps, batch, seed=2024, kappa=1.0,
     ctx = capi.Context(wl["net"], wl["topos"], wl["x0"], eps=wl["eps"], nc=wl["nc"], seed=seed,
                        waves_per_block=wpb, blocks_per_cu=bpc)
     ctx.mlp_set(w, norm)
-    ctx.set_tau_mode(True, kappa=kappa, theta=3.0, viol_rate_max=vmax)
+    ctx.set_tau_mode(True, kappa=kappa, theta=learned.THETA, viol_rate_max=vmax)
     ctx.arm(t_end=float("inf"))
     ctx.iterate(warmup)
     ctx.sync()

@@ -173,16 +173,22 @@ int plis_redo_tiles(plis_ctx *ctx, uint64_t *total);
 int plis_autotune(plis_ctx *ctx, uint32_t iters);
 
 /* ---- learned tau proposal (no counterpart in the reference) ----
- * A 12-64-64-2 MLP (bf16 weights, fp32 accumulation on matrix cores) maps
- * local features of each subvolume to a log-tau proposal and a critical-flag
- * logit.  w32 = 5378 fp32 parameters W1[16][64] b1 W2[64][64] b2 W3[64][2]
- * b3 (rows 12..15 of W1 unused); norm = feature means[12], scales[12], target
- * mean, target scale (log tau standardisation). */
+ * A 12-64-64-2 MLP (bf16 weights, fp32 accumulation on matrix cores; GELU
+ * activations by default, ReLU selectable) maps local features of each
+ * subvolume to a log-tau proposal and a critical-flag logit.  w32 = 5378
+ * fp32 parameters W1[16][64] b1 W2[64][64] b2 W3[64][2] b3 (rows 12..15 of
+ * W1 unused); norm = feature means[12], scales[12], target mean, target
+ * scale (log tau standardisation). */
 int plis_mlp_set(plis_ctx *ctx, const float *w32, const float *norm);
 int plis_mlp_get(plis_ctx *ctx, float *w32);
+/* activation of the network's hidden layers: 0 = ReLU, 1 = GELU (default;
+ * evaluated as max(z,0) - |z| max(0, p(|z|)) with a fitted cubic p, within
+ * 0.006 of the erf form everywhere) */
+int plis_mlp_activation(plis_ctx *ctx, int32_t act);
 /* outputs [n][2] (standardised log tau, critical logit) for feature rows [n][12] */
 int plis_mlp_forward(plis_ctx *ctx, const float *feats, uint32_t n, float *out);
-/* mean loss and its gradient over n (multiple of 64) records, no update */
+/* mean loss and its gradient over n (multiple of 64) records, no update;
+ * loss per record = (o0 - ylog)^2 + softplus(o1) - ycrit o1 */
 int plis_mlp_loss_grad(plis_ctx *ctx, const float *feats, const float *ylog, const float *ycrit,
                        uint32_t n, float *grad, double *loss);
 /* `steps` Adam steps (lr, beta 0.9/0.999) on random minibatches of `batch`
@@ -190,16 +196,27 @@ int plis_mlp_loss_grad(plis_ctx *ctx, const float *feats, const float *ylog, con
 int plis_mlp_train(plis_ctx *ctx, const float *feats, const float *ylog, const float *ycrit,
                    uint64_t n, uint32_t steps, uint32_t batch, float lr, uint64_t seed,
                    float *losses);
-/* training records of realization r's current state: features [N][12],
- * log of Cao's tau' per subvolume [N], critical flag (critical propensity
- * > 0) [N] */
+/* Training records of realization r's current state: features [N][12],
+ * log of Cao's tau'_v [N], and the accept/reject label [N] of a trial leap
+ * of subvolume v alone at tau'_v, drawn from its own streams (1 = the
+ * post-leap leap condition below breaks at v: reject; 0 = accept).  The
+ * condition's tolerance is the context's theta (plis_set_tau_mode). */
 int plis_records(plis_ctx *ctx, int32_t r, float *feats, float *ylog, float *crit);
-/* learned = 1: tau = kappa * min_v exp(network log tau) (the critical part
- * stays exact); a step is kept only if no population goes negative and at
- * most mu + 3 sqrt(mu) + 2 subvolumes, mu = viol_rate_max * N, exceed
- * theta * max(eps x / g, 1) in realized change (a Poisson upper bound around
- * the plain method's violation rate), otherwise it is redone with Cao's tau.
- * theta also sets the bound the violation statistics use in plain mode. */
+/* Post-leap leap condition (Cao, Gillespie & Petzold 2006 eq. 5 in
+ * propensity form with its one-molecule floor, checked after the leap):
+ * subvolume v breaks it when a non-critical channel j changes by
+ * |a_j(x') - a_j(x)| > theta max(eps, max_i r_ij / x_i) a_j(x), or an active
+ * hop channel of species s by |x'_s - x_s| > theta max(eps x_s, 1).
+ * theta = 0 turns the condition off (plain runs); theta > 0 counts the
+ * subvolumes that break it into plis_stats.violations.
+ * learned = 1 (needs theta > 0 and a network): tau = min_v tau_v with
+ * tau_v = kappa exp(network log tau), halved where the critical logit is
+ * positive (the critical part stays exact); the proposals of the next step
+ * are computed inside the leap pass from the state it produces.  A learned
+ * step is kept only if no population goes negative and at most
+ * mu + 3 sqrt(mu) + 2 subvolumes break the condition, mu = viol_rate_max N
+ * (a Poisson upper bound around the plain method's own rate), otherwise it
+ * is redone with Cao's tau and fresh draws (counted in learned_rejects). */
 int plis_set_tau_mode(plis_ctx *ctx, int32_t learned, double kappa, double theta,
                       double viol_rate_max);
 

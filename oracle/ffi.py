@@ -97,10 +97,12 @@ def _declare_mlp(L):
     P = C.POINTER
     L.orc_mlp_param_count.restype = C.c_int
     L.orc_mlp_forward.argtypes = [P(C.c_float), P(C.c_float), P(C.c_float), C.c_int32, C.c_int32,
-                                  P(C.c_float)]
+                                  C.c_int32, P(C.c_float)]
     L.orc_mlp_forward.restype = None
     L.orc_mlp_loss_grad.argtypes = [P(C.c_float), P(C.c_float), P(C.c_float), C.c_int32, C.c_int32,
-                                    P(C.c_float), P(C.c_float), P(C.c_double)]
+                                    C.c_int32, P(C.c_float), P(C.c_float), P(C.c_double)]
+    L.orc_gelu.argtypes = [C.c_float]
+    L.orc_gelu.restype = C.c_float
     L.orc_mlp_loss_grad.restype = C.c_double
     L.orc_adam_step.argtypes = [P(C.c_float), P(C.c_float), P(C.c_float), P(C.c_float), C.c_int32,
                                 C.c_int32, C.c_float, C.c_float, C.c_float, C.c_float]
@@ -109,6 +111,9 @@ def _declare_mlp(L):
                                P(C.c_float)]
     L.orc_violations.argtypes = [P(Network), P(Topology), P(Params), P(C.c_uint32),
                                  P(C.c_uint32), C.c_double, P(C.c_uint8)]
+    L.orc_records.argtypes = [P(Network), P(Topology), P(Params), P(C.c_uint32), C.c_uint32,
+                              C.c_uint32, C.c_float, C.c_double, P(C.c_float), P(C.c_float),
+                              P(C.c_float)]
 
 
 def _ptr(a, ct):
@@ -308,17 +313,38 @@ def violations(model, x, xn, eps, nc, theta):
     return n, flags
 
 
-def mlp_forward(w, norm, feats, bf16=True):
+def records(model, x, eps, nc, seed, theta=1.0, realization=0, step=0, prev_logtau=0.0):
+    """Training records of state x: features [N][12], log Cao tau' [N] and
+    the trial-leap reject label [N]."""
+    x = np.ascontiguousarray(x, dtype=np.uint32)
+    N = model.N
+    f = np.empty((N, 12), np.float32)
+    y = np.empty(N, np.float32)
+    lab = np.empty(N, np.float32)
+    p = Params(eps, nc, seed)
+    rc = lib().orc_records(C.byref(model.net), C.byref(model.topo), C.byref(p), _ptr(x, C.c_uint32),
+                           realization, step, prev_logtau, theta, _ptr(f, C.c_float), _ptr(y, C.c_float),
+                           _ptr(lab, C.c_float))
+    assert rc == 0
+    return f, y, lab
+
+
+def gelu(z):
+    """The networks' GELU (single precision), elementwise."""
+    return np.array([lib().orc_gelu(float(v)) for v in np.ravel(z)], np.float32).reshape(np.shape(z))
+
+
+def mlp_forward(w, norm, feats, bf16=True, act=1):
     w = np.ascontiguousarray(w, dtype=np.float32)
     norm = np.ascontiguousarray(norm, dtype=np.float32)
     feats = np.ascontiguousarray(feats, dtype=np.float32)
     out = np.empty((feats.shape[0], 2), dtype=np.float32)
     lib().orc_mlp_forward(_ptr(w, C.c_float), _ptr(norm, C.c_float), _ptr(feats, C.c_float),
-                          feats.shape[0], 1 if bf16 else 0, _ptr(out, C.c_float))
+                          feats.shape[0], 1 if bf16 else 0, act, _ptr(out, C.c_float))
     return out
 
 
-def mlp_loss_grad(w, norm, feats, ylog, ycrit, bf16=True):
+def mlp_loss_grad(w, norm, feats, ylog, ycrit, bf16=True, act=1):
     w = np.ascontiguousarray(w, dtype=np.float32)
     norm = np.ascontiguousarray(norm, dtype=np.float32)
     feats = np.ascontiguousarray(feats, dtype=np.float32)
@@ -326,7 +352,7 @@ def mlp_loss_grad(w, norm, feats, ylog, ycrit, bf16=True):
     ycrit = np.ascontiguousarray(ycrit, dtype=np.float32)
     g = np.empty(mlp_param_count(), dtype=np.float64)
     loss = lib().orc_mlp_loss_grad(_ptr(w, C.c_float), _ptr(norm, C.c_float), _ptr(feats, C.c_float),
-                                   feats.shape[0], 1 if bf16 else 0, _ptr(ylog, C.c_float),
+                                   feats.shape[0], 1 if bf16 else 0, act, _ptr(ylog, C.c_float),
                                    _ptr(ycrit, C.c_float), _ptr(g, C.c_double))
     return loss, g
 

@@ -59,7 +59,7 @@ static const double LOGFACT[10] = {
 #define C_1_1260 0x1.a01a01a01a01ap-11
 
 /* Philox stream purposes (3 bits of counter word 3) */
-enum { P_TAU2 = 1, P_CRIT = 2, P_REACT = 3, P_HOP = 4, P_EXTRA = 5, P_SSA = 6 };
+enum { P_TAU2 = 1, P_CRIT = 2, P_REACT = 3, P_HOP = 4, P_EXTRA = 5, P_SSA = 6, P_TRIAL = 7 };
 #define SELF_ID 0xFFFFFFFFu
 #define MAX_ATTEMPT 63u
 #define POISSON_INV_MAX 10.0
@@ -1057,9 +1057,53 @@ int orc_features(const orc_network *net, const orc_topology *topo, const orc_par
     return 0;
 }
 
-/* Post-leap leap-condition check (Anderson 2008 form of Cao's bound): the
- * realized change of every reactant species must stay within
- * theta * max(eps x / g, 1).  flags[v] = 1 where it does not. */
+/* Post-leap leap condition in propensity form (Cao, Gillespie & Petzold
+ * 2006, eq. 5, with their one-molecule floor; checked after the leap as in
+ * Anderson 2008): every non-critical channel j of v with a_j(x) > 0 must
+ * satisfy
+ *     |a_j(x') - a_j(x)| <= theta * max(eps, phi_j(x)) * a_j(x),
+ * phi_j(x) = max over reactants i of r_ij / x_i, the relative change one
+ * molecule of i makes in a_j; a zero-order channel never changes.  An active
+ * hop channel of species s (propensity d_s x_s w_e) changes by the factor
+ * x'_s / x_s, so its condition is |x'_s - x_s| <= theta * max(eps x_s, 1).
+ * nx holds v's S new counts (any sign); a negative count breaks it too. */
+#define LC_SLACK (1.0 + 0x1p-30)
+
+static int leap_condition_broken(const model_t *m, const csr_t *g, const orc_params *p,
+                                 const uint32_t *x, uint32_t v, const int64_t *nx, double theta)
+{
+    double a[128];
+    int crit[128];
+    uint32_t xn[64];
+    for (int s = 0; s < m->S; s++) {
+        if (nx[s] < 0 || nx[s] > 0xFFFFFFFFll) return 1;
+        xn[s] = (uint32_t)nx[s];
+    }
+    react_props(m, g, p, x, v, a, crit);
+    int c = comp_of(g, v);
+    const double eps = p->epsilon;
+    for (int j = 0; j < m->R; j++) {
+        if (crit[j] || !(a[j] > 0.0) || m->nrs[j] == 0) continue;
+        double aj = m->k[c * m->R + j];
+        for (int i = 0; i < m->nrs[j]; i++) aj = aj * binom_d(xn[m->rs[j * 3 + i]], m->rr[j * 3 + i]);
+        /* D > theta max(eps, max_i r_i / x_i) a_j, without divides; the
+         * relative slack 2^-30 keeps a change that meets a bound exactly (a
+         * one-molecule change at the floor) on the holding side whatever the
+         * rounding of the products */
+        double D = fabs(aj - a[j]), t = (theta * a[j]) * LC_SLACK;
+        int over = D > t * eps;
+        for (int i = 0; i < m->nrs[j]; i++)
+            over = over && D * (double)X(x, g->N, m->rs[j * 3 + i], v) > t * (double)m->rr[j * 3 + i];
+        if (over) return 1;
+    }
+    for (int s = 0; s < m->S; s++) {
+        uint32_t xs = X(x, g->N, s, v);
+        if (!hop_active(m, p, m->d[c * m->S + s], xs)) continue;
+        if (fabs((double)xn[s] - (double)xs) > (theta * fmax(eps * (double)xs, 1.0)) * LC_SLACK) return 1;
+    }
+    return 0;
+}
+
 int orc_violations(const orc_network *net, const orc_topology *topo, const orc_params *p,
                    const uint32_t *x, const uint32_t *xn, double theta, uint8_t *flags)
 {
@@ -1070,19 +1114,87 @@ int orc_violations(const orc_network *net, const orc_topology *topo, const orc_p
 #pragma omp parallel for schedule(static) reduction(+ : count)
     for (int64_t vv = 0; vv < (int64_t)g.N; vv++) {
         uint32_t v = (uint32_t)vv;
-        int bad = 0;
-        for (int s = 0; s < m.S; s++) {
-            if (m.hor[s] == 0) continue;
-            uint32_t a = X(x, g.N, s, v), b = X(xn, g.N, s, v);
-            double dx = fabs((double)b - (double)a);
-            double bound = theta * fmax((p->epsilon * (double)a) / g_cao(&m, s, a), 1.0);
-            if (dx > bound) bad = 1;
-        }
+        int64_t nx[64];
+        for (int s = 0; s < m.S; s++) nx[s] = X(xn, g.N, s, v);
+        int bad = leap_condition_broken(&m, &g, p, x, v, nx, theta);
         flags[v] = (uint8_t)bad;
         count += bad;
     }
     model_free(&m); csr_free(&g);
     return (int)count;
+}
+
+/* Training records of state x: per subvolume the features, log of Cao's
+ * tau'_v (capped at 1e6) and the accept/reject label of a trial leap of v
+ * alone with tau = tau'_v: its non-critical reactions, out-hops and in-hops
+ * (at the neighbours' propensities) are drawn from the TRIAL streams
+ * (counter (v, SELF_ID or u, step, purpose TRIAL), block 96 + j/4 for
+ * reactions, 64 + s/2 for hops, word 2(s&1) + 0 out / 1 in), and label = 1 where the
+ * leap condition above is broken at v (reject), 0 where it holds.  A
+ * subvolume without non-critical channels (tau' infinite) is labelled 0. */
+int orc_records(const orc_network *net, const orc_topology *topo, const orc_params *p,
+                const uint32_t *x, uint32_t realization, uint32_t step, float prev_logtau,
+                double theta, float *feats, float *ylog, float *label)
+{
+    if (orc_features(net, topo, p, x, prev_logtau, feats)) return -1;
+    model_t m; csr_t g;
+    if (model_build(net, &m)) { model_free(&m); return -1; }
+    if (csr_build(topo, &g)) { model_free(&m); csr_free(&g); return -1; }
+    const uint32_t key[2] = {(uint32_t)p->seed, (uint32_t)(p->seed >> 32)};
+    const uint32_t N = g.N;
+#pragma omp parallel for schedule(static)
+    for (int64_t vv = 0; vv < (int64_t)N; vv++) {
+        uint32_t v = (uint32_t)vv;
+        double tp, a0c;
+        tau_v(&m, &g, p, x, v, &tp, &a0c);
+        ylog[v] = tp < 1e6 ? (float)orc_log(tp) : (float)orc_log(1e6);
+        if (!(tp < INFINITY)) { label[v] = 0.0f; continue; }
+        double a[128];
+        int crit[128];
+        int64_t nx[64];
+        react_props(&m, &g, p, x, v, a, crit);
+        int c = comp_of(&g, v);
+        for (int s = 0; s < m.S; s++) nx[s] = X(x, N, s, v);
+        for (int j = 0; j < m.R; j++) {
+            if (crit[j] || !(a[j] > 0.0)) continue;
+            uint32_t cc[4] = {v, SELF_ID, step, ctr3(realization, 0, P_TRIAL, 96u + ((uint32_t)j >> 2))};
+            uint32_t o[4];
+            orc_philox4x32_10(cc, key, o);
+            int64_t k = orc_poisson(tp * a[j], o[j & 3], cc, key, (uint32_t)j & 3u);
+            for (int s = 0; s < m.S; s++) nx[s] += k * (int64_t)m.nu[s * m.R + j];
+        }
+        for (int s = 0; s < m.S; s++) {
+            uint32_t xs = X(x, N, s, v);
+            double d = m.d[c * m.S + s];
+            if (!hop_active(&m, p, d, xs)) continue;
+            double dx = d * (double)xs;
+            for (uint32_t q = g.rp[v]; q < g.rp[v + 1]; q++) {
+                uint32_t u = g.col[q];
+                uint32_t wd = 2u * ((uint32_t)s & 1u);
+                uint32_t cc[4] = {v, u, step, ctr3(realization, 0, P_TRIAL, 64u + ((uint32_t)s >> 1))};
+                uint32_t o[4];
+                orc_philox4x32_10(cc, key, o);
+                nx[s] -= orc_poisson(tp * (dx * (double)g.w[q]), o[wd], cc, key, wd);
+            }
+        }
+        for (uint32_t q = g.irp[v]; q < g.irp[v + 1]; q++) {
+            uint32_t u = g.isrc[q];
+            int cu = comp_of(&g, u);
+            for (int s = 0; s < m.S; s++) {
+                uint32_t xu = X(x, N, s, u);
+                double du = m.d[cu * m.S + s];
+                if (!hop_active(&m, p, du, xu)) continue;
+                uint32_t wd = 2u * ((uint32_t)s & 1u) + 1u;
+                uint32_t cc[4] = {v, u, step, ctr3(realization, 0, P_TRIAL, 64u + ((uint32_t)s >> 1))};
+                uint32_t o[4];
+                orc_philox4x32_10(cc, key, o);
+                nx[s] += orc_poisson(tp * ((du * (double)xu) * (double)g.iw[q]), o[wd], cc, key, wd);
+            }
+        }
+        label[v] = leap_condition_broken(&m, &g, p, x, v, nx, theta) ? 1.0f : 0.0f;
+    }
+    model_free(&m); csr_free(&g);
+    return 0;
 }
 
 int orc_num_threads(void)

@@ -118,13 +118,22 @@ int orc_num_threads(void);
 /* ---- learned tau (plis_mlp.c) ---- */
 int orc_features(const orc_network *net, const orc_topology *topo, const orc_params *p,
                  const uint32_t *x, float prev_logtau, float *feats /* N*12 */);
+/* post-leap leap condition (propensity form) per subvolume; returns the
+ * number of subvolumes where it is broken */
 int orc_violations(const orc_network *net, const orc_topology *topo, const orc_params *p,
                    const uint32_t *x, const uint32_t *xn, double theta, uint8_t *flags);
+/* training records: features, log Cao tau', trial-leap reject label */
+int orc_records(const orc_network *net, const orc_topology *topo, const orc_params *p,
+                const uint32_t *x, uint32_t realization, uint32_t step, float prev_logtau,
+                double theta, float *feats, float *ylog, float *label);
 int orc_mlp_param_count(void);
+/* act: 0 = ReLU, 1 = GELU (the clamped-cubic form of plis_mlp.c) */
 void orc_mlp_forward(const float *w, const float *norm, const float *feats, int32_t n,
-                     int32_t bfon, float *out);
+                     int32_t bfon, int32_t act, float *out);
 double orc_mlp_loss_grad(const float *w, const float *norm, const float *feats, int32_t n,
-                         int32_t bfon, const float *ylog, const float *ycrit, double *grad);
+                         int32_t bfon, int32_t act, const float *ylog, const float *ycrit,
+                         double *grad);
+float orc_gelu(float z);
 void orc_adam_step(float *w, float *m, float *v, const float *g, int32_t count, int32_t t,
                    float lr, float b1, float b2, float eps);
 

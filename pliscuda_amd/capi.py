@@ -23,7 +23,8 @@ EXPORTS = ("plis_create", "plis_run", "plis_run_until", "plis_arm", "plis_iterat
            "plis_set_state", "plis_get_state", "plis_get_stats", "plis_sync", "plis_debug_tau",
            "plis_launch_shape", "plis_redo_tiles", "plis_checkpoint_size", "plis_checkpoint", "plis_restore", "plis_autotune", "plis_destroy", "plis_last_error",
            "plis_device_count", "plis_mlp_set", "plis_mlp_get", "plis_mlp_forward",
-           "plis_mlp_loss_grad", "plis_mlp_train", "plis_records", "plis_set_tau_mode")
+           "plis_mlp_loss_grad", "plis_mlp_train", "plis_records", "plis_set_tau_mode",
+           "plis_mlp_activation")
 MLP_PARAMS = 5378
 
 
@@ -104,6 +105,7 @@ def _declare(L):
     L.plis_mlp_train.argtypes = [vp, fp, fp, fp, C.c_uint64, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64, fp]
     L.plis_records.argtypes = [vp, C.c_int32, fp, fp, fp]
     L.plis_set_tau_mode.argtypes = [vp, C.c_int32, C.c_double, C.c_double, C.c_double]
+    L.plis_mlp_activation.argtypes = [vp, C.c_int32]
 
 
 def _ptr(a, ct):
@@ -301,7 +303,11 @@ class Context:
                   "plis_records")
         return f, y, c
 
-    def set_tau_mode(self, learned, kappa=1.0, theta=3.0, viol_rate_max=1.0):
+    def mlp_activation(self, act):
+        """0 = ReLU, 1 = GELU (default)."""
+        self._chk(lib().plis_mlp_activation(self.h, int(act)), "plis_mlp_activation")
+
+    def set_tau_mode(self, learned, kappa=1.0, theta=1.0, viol_rate_max=1.0):
         self._chk(lib().plis_set_tau_mode(self.h, 1 if learned else 0, kappa, theta, viol_rate_max),
                   "plis_set_tau_mode")
 

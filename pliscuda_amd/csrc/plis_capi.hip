@@ -66,9 +66,10 @@ struct plis_ctx {
     // captured step sequences of 1, 2, 4, ..., 64 iterations (index log2 n)
     hipGraphExec_t graphs[kGraphSizes] = {};
     std::vector<void*> allocs;
-    // post-leap check and learned tau
-    double theta = 3.0, viol_rate_max = 1.0, kappa = 1.0;
+    // post-leap leap condition (theta = 0: not evaluated) and learned tau
+    double theta = 0.0, viol_rate_max = 1.0, kappa = 1.0;
     int learned_on = 0;
+    int mlp_act = 1;  // 0 ReLU, 1 GELU
     float *d_w32 = nullptr, *d_m = nullptr, *d_v = nullptr, *d_norm = nullptr, *d_grad = nullptr;
     __bf16* d_q = nullptr;
     float* d_partial = nullptr;
@@ -303,6 +304,9 @@ static KernelArgs make_args(plis_ctx* c) {
     a.viol_rate_max = c->viol_rate_max;
     a.kappa = c->kappa;
     a.learned_on = c->learned_on;
+    a.judge_on = c->theta > 0.0 ? 1 : 0;
+    a.props_in_finish = (c->learned_on && c->edge_parallel) ? 1 : 0;
+    a.mlp_act = c->mlp_act;
     a.mlp_w32 = c->d_w32;
     a.mlp_q = c->d_q;
     a.mlp_norm = c->d_norm;
@@ -313,15 +317,22 @@ static const int kVariantS[] = {1, 3, 5, 8, 64};
 static const int kVariantR[] = {4, 4, 8, 16, 128};
 
 // Kernels of one step: 0 tau pass (learned and/or Cao), 1 decide, 2 leap
-// pass, 3 commit; 5 = training records of one realization.
+// pass, 3 commit; 5 = training records of one realization.  In a learned
+// context the end of the edge-parallel leap pass also proposes the next
+// step's tau (learned_finish_kernel); learned_tau_kernel covers the steps
+// whose proposals it could not make.
 template <int MS, int MR>
 static void launch_variant(plis_ctx* c, int which, const KernelArgs& a) {
     const dim3 grid(c->blocks), block(64 * c->wpb), rgrid(c->nreal), rblock(1024), dblock(decide_threads<MS>());
     const bool mc = c->nh.C > 1;  // several compartments: per-subvolume rate tables
+    const bool gelu = c->mlp_act == 1;
     switch (which) {
         case 0:
             if constexpr (MS <= 8) {
-                if (c->learned_on) hipLaunchKernelGGL((learned_tau_kernel<MS, MR>), grid, block, 0, c->stream, a);
+                if (c->learned_on) {
+                    if (gelu) hipLaunchKernelGGL((learned_tau_kernel<MS, MR, 1>), grid, block, 0, c->stream, a);
+                    else hipLaunchKernelGGL((learned_tau_kernel<MS, MR, 0>), grid, block, 0, c->stream, a);
+                }
             }
             if (mc) hipLaunchKernelGGL((phase_a_kernel<MS, MR, true>), grid, block, 0, c->stream, a);
             else hipLaunchKernelGGL((phase_a_kernel<MS, MR, false>), grid, block, 0, c->stream, a);
@@ -337,13 +348,23 @@ static void launch_variant(plis_ctx* c, int which, const KernelArgs& a) {
                         if (c->split_ev) (void)hipEventRecord(c->split_ev[0], c->stream);
                         hipLaunchKernelGGL((phase_b_redo_kernel<MS, MR, true>), grid, block, 0, c->stream, a);
                         if (c->split_ev) (void)hipEventRecord(c->split_ev[1], c->stream);
-                        hipLaunchKernelGGL((phase_b_finish_kernel<MS, MR, true>), grid, block, 0, c->stream, a);
+                        if (!c->learned_on)
+                            hipLaunchKernelGGL((phase_b_finish_kernel<MS, MR, true>), grid, block, 0, c->stream, a);
+                        else if (gelu)
+                            hipLaunchKernelGGL((learned_finish_kernel<MS, MR, true, 1>), grid, block, 0, c->stream, a);
+                        else
+                            hipLaunchKernelGGL((learned_finish_kernel<MS, MR, true, 0>), grid, block, 0, c->stream, a);
                     } else {
                         hipLaunchKernelGGL((phase_b_kernel<MS, MR, true, false>), grid, block, 0, c->stream, a);
                         if (c->split_ev) (void)hipEventRecord(c->split_ev[0], c->stream);
                         hipLaunchKernelGGL((phase_b_redo_kernel<MS, MR, false>), grid, block, 0, c->stream, a);
                         if (c->split_ev) (void)hipEventRecord(c->split_ev[1], c->stream);
-                        hipLaunchKernelGGL((phase_b_finish_kernel<MS, MR, false>), grid, block, 0, c->stream, a);
+                        if (!c->learned_on)
+                            hipLaunchKernelGGL((phase_b_finish_kernel<MS, MR, false>), grid, block, 0, c->stream, a);
+                        else if (gelu)
+                            hipLaunchKernelGGL((learned_finish_kernel<MS, MR, false, 1>), grid, block, 0, c->stream, a);
+                        else
+                            hipLaunchKernelGGL((learned_finish_kernel<MS, MR, false, 0>), grid, block, 0, c->stream, a);
                     }
                     break;
                 }
@@ -661,6 +682,17 @@ static int pull_ctl(plis_ctx* c) {
     return 0;
 }
 
+// The tile partials no longer hold proposals for the current state (the
+// host changed the state, the network or the tau mode).
+static int invalidate_props(plis_ctx* c) {
+    if (int rc = pull_ctl(c)) return rc;
+    for (auto& q : c->ctl_host) q.prop_ok = 0u;
+    HIPCHK(c, hipMemcpyAsync(c->d_ctl, c->ctl_host.data(), c->ctl_host.size() * sizeof(RealCtl),
+                             hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
 // Arm every realization for (t_end, t_pause, step_limit); returns how many
 // are already done through *ndone_out.
 static int arm(plis_ctx* c, double t_end, double t_pause, uint64_t step_limit, uint32_t* ndone_out) {
@@ -810,7 +842,7 @@ extern "C" int plis_set_state(plis_ctx* c, int32_t r, const uint32_t* x) {
     const RealInfo& ri = c->info[r];
     uint32_t* X = c->ctl_host[r].cur ? c->d_x1 : c->d_x0;
     HIPCHK(c, hipMemcpy(X + ri.x_off, x, (size_t)c->S * ri.N * sizeof(uint32_t), hipMemcpyHostToDevice));
-    return PLIS_OK;
+    return invalidate_props(c);
 }
 
 extern "C" int plis_get_state(plis_ctx* c, int32_t r, uint32_t* x) {
@@ -858,7 +890,7 @@ extern "C" int plis_checkpoint(plis_ctx* c, void* buf, uint64_t bytes) {
     char* p = static_cast<char*>(buf);
     CkptHeader h{};
     std::memcpy(h.magic, kCkptMagic, sizeof h.magic);
-    h.version = 2u;
+    h.version = 3u;
     h.nreal = c->nreal;
     h.S = c->S;
     h.R = c->R;
@@ -894,7 +926,7 @@ extern "C" int plis_restore(plis_ctx* c, const void* buf, uint64_t bytes) {
         return PLIS_ERR_ARG;
     }
     std::memcpy(&h, p, sizeof h);
-    if (std::memcmp(h.magic, kCkptMagic, sizeof h.magic) != 0 || h.version != 2u ||
+    if (std::memcmp(h.magic, kCkptMagic, sizeof h.magic) != 0 || h.version != 3u ||
         h.ctl_bytes != sizeof(RealCtl)) {
         c->err = "plis_restore: not a checkpoint of this library version";
         return PLIS_ERR_ARG;
@@ -922,7 +954,10 @@ extern "C" int plis_restore(plis_ctx* c, const void* buf, uint64_t bytes) {
     std::vector<RealCtl> ctl(c->nreal);
     std::memcpy(ctl.data(), p, c->nreal * sizeof(RealCtl));
     p += c->nreal * sizeof(RealCtl);
-    for (auto& q : ctl) q.cur = 0u;  // counts go to buffer 0
+    for (auto& q : ctl) {
+        q.cur = 0u;  // counts go to buffer 0
+        q.prop_ok = 0u;  // the blob holds no tile partials
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipMemcpy(c->d_x0, p, c->total_counts * sizeof(uint32_t), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_ctl, ctl.data(), ctl.size() * sizeof(RealCtl), hipMemcpyHostToDevice));
@@ -1027,7 +1062,7 @@ static int mlp_alloc(plis_ctx* c) {
     std::vector<float> zero(kPTotal + 1, 0.0f), norm(26, 0.0f);
     for (int k = 12; k < 24; ++k) norm[k] = 1.0f;
     norm[25] = 1.0f;
-    c->train_blocks = 512;
+    c->train_blocks = 1024;  // one 64-sample tile per block at batch 65,536
     if ((rc = upload(c, &c->d_w32, zero)) || (rc = upload(c, &c->d_m, zero)) || (rc = upload(c, &c->d_v, zero)) ||
         (rc = upload(c, &c->d_grad, zero)) || (rc = upload(c, &c->d_norm, norm)) ||
         (rc = dev_alloc(c, &c->d_q, (size_t)kQTotal)) ||
@@ -1061,7 +1096,7 @@ extern "C" int plis_mlp_set(plis_ctx* c, const float* w32, const float* norm) {
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->adam_t = 0;
-    return PLIS_OK;
+    return invalidate_props(c);
 }
 
 extern "C" int plis_mlp_get(plis_ctx* c, float* w32) {
@@ -1088,8 +1123,12 @@ extern "C" int plis_mlp_forward(plis_ctx* c, const float* feats, uint32_t n, flo
     if (e == hipSuccess) {
         const MlpDev m{c->d_w32, c->d_q, c->d_norm};
         const uint32_t tiles = (n + 63) / 64;
-        hipLaunchKernelGGL(mlp_infer_kernel, dim3(std::min<uint32_t>(tiles, 4096)), dim3(64), 0, c->stream, m, d_f,
-                           n, d_o);
+        if (c->mlp_act == 1)
+            hipLaunchKernelGGL(mlp_infer_kernel<1>, dim3(std::min<uint32_t>(tiles, 4096)), dim3(64), 0, c->stream, m,
+                               d_f, n, d_o);
+        else
+            hipLaunchKernelGGL(mlp_infer_kernel<0>, dim3(std::min<uint32_t>(tiles, 4096)), dim3(64), 0, c->stream, m,
+                               d_f, n, d_o);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -1138,7 +1177,8 @@ static void train_launch(plis_ctx* c, uint64_t n, uint32_t batch, uint32_t step,
     ta.full_pass = full;
     ta.partial = c->d_partial;
     const uint32_t blocks = std::min<uint32_t>(c->train_blocks, batch / 64);
-    hipLaunchKernelGGL(mlp_train_kernel, dim3(blocks), dim3(64), 0, c->stream, ta);
+    if (c->mlp_act == 1) hipLaunchKernelGGL(mlp_train_kernel<1>, dim3(blocks), dim3(64), 0, c->stream, ta);
+    else hipLaunchKernelGGL(mlp_train_kernel<0>, dim3(blocks), dim3(64), 0, c->stream, ta);
     AdamArgs aa{c->d_w32, c->d_m, c->d_v, c->d_grad, c->d_partial, blocks, adam_t, lr, 0.9f, 0.999f, 1e-8f};
     hipLaunchKernelGGL(mlp_adam_kernel, dim3((kPTotal + 1 + kAdamParams - 1) / kAdamParams), dim3(256), 0,
                        c->stream, aa);
@@ -1196,7 +1236,7 @@ extern "C" int plis_mlp_train(plis_ctx* c, const float* feats, const float* ylog
         c->err = std::string("plis_mlp_train: ") + hipGetErrorString(e);
         return PLIS_ERR_HIP;
     }
-    return PLIS_OK;
+    return invalidate_props(c);
 }
 
 extern "C" int plis_records(plis_ctx* c, int32_t r, float* feats, float* ylog, float* crit) {
@@ -1232,7 +1272,11 @@ extern "C" int plis_records(plis_ctx* c, int32_t r, float* feats, float* ylog, f
 }
 
 extern "C" int plis_set_tau_mode(plis_ctx* c, int32_t learned, double kappa, double theta, double viol_rate_max) {
-    if (!c || !(kappa > 0.0) || !(theta > 0.0) || !(viol_rate_max >= 0.0)) return PLIS_ERR_ARG;
+    if (!c || !(kappa > 0.0) || !(theta >= 0.0) || !(viol_rate_max >= 0.0)) return PLIS_ERR_ARG;
+    if (learned && !(theta > 0.0)) {
+        c->err = "learned tau needs the judge: theta > 0";
+        return PLIS_ERR_ARG;
+    }
     if (learned && (c->variant > 3 || !c->d_w32)) {
         c->err = c->d_w32 ? "learned tau supports networks of at most 8 species and 16 reactions"
                           : "set the network (plis_mlp_set) before enabling learned tau";
@@ -1244,9 +1288,19 @@ extern "C" int plis_set_tau_mode(plis_ctx* c, int32_t learned, double kappa, dou
     c->viol_rate_max = viol_rate_max;
     drop_graphs(c);
     if (int rc = pull_ctl(c)) return rc;
-    for (auto& q : c->ctl_host) q.fallback = 0u;
+    for (auto& q : c->ctl_host) {
+        q.fallback = 0u;
+        q.prop_ok = 0u;
+    }
     HIPCHK(c, hipMemcpyAsync(c->d_ctl, c->ctl_host.data(), c->ctl_host.size() * sizeof(RealCtl),
                              hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return PLIS_OK;
+}
+
+extern "C" int plis_mlp_activation(plis_ctx* c, int32_t act) {
+    if (!c || (act != 0 && act != 1)) return PLIS_ERR_ARG;
+    c->mlp_act = act;
+    drop_graphs(c);
+    return invalidate_props(c);
 }

@@ -151,31 +151,6 @@ __device__ __forceinline__ double eps_x_over_g(double eps, int hor, int mult, ui
     return ex / g_cao(hor, mult, xs);
 }
 
-// Post-leap check (Anderson 2008 form of Cao's leap condition): a reactant
-// species whose realized change exceeds theta * max(eps x / g, 1) (species
-// of highest order 0 are free).
-__device__ __forceinline__ bool species_violates(int hor, int mult, uint32_t x, long long nx, double eps,
-                                                 double theta) {
-    if (hor == 0) return false;
-    const double dx = __builtin_fabs((double)nx - (double)x);
-    const double bound = theta * fmax(eps_x_over_g(eps, hor, mult, x), 1.0);
-    return dx > bound;
-}
-
-// Any violating species at one subvolume.
-template <int MS>
-__device__ __forceinline__ uint32_t leap_violation(cptr<int32_t> hor, cptr<int32_t> mult, int S,
-                                                   const uint32_t (&x)[MS], const long long (&nx)[MS],
-                                                   double eps, double theta) {
-    uint32_t bad = 0;
-#pragma unroll
-    for (int s = 0; s < MS; ++s) {
-        if (s >= S) break;
-        if (species_violates(hor[s], mult[s], x[s], nx[s], eps, theta)) bad = 1u;
-    }
-    return bad;
-}
-
 // Reaction propensities a_j = k_j prod_s C(x_s, r_sj) and the critical flags
 // (Cao: L_j = min_s floor(x_s / r_sj) < n_c) at one subvolume.
 template <int MR, int MS>
@@ -207,6 +182,70 @@ __device__ __forceinline__ void react_props(const NetView& n, const uint32_t (&x
 template <int MR>
 __device__ __forceinline__ bool is_crit(const uint32_t (&crit)[(MR + 31) / 32], int j) {
     return (crit[j >> 5] >> (j & 31)) & 1u;
+}
+
+// Post-leap leap condition in propensity form (Cao, Gillespie & Petzold
+// 2006, eq. 5, with their one-molecule floor; checked after the leap as in
+// Anderson 2008).  A non-critical channel j with a_j(x) > 0 breaks it when
+//     |a_j(x') - a_j(x)| > theta max(eps, phi_j) a_j(x),
+// phi_j = max over reactants i of r_ij / x_i (one molecule's relative
+// effect); evaluated without divides as D > (theta a_j) eps and
+// D x_i > (theta a_j) r_ij for every reactant i.  An active hop channel of
+// species s breaks it when |x'_s - x_s| > theta max(eps x_s, 1).  Bounds
+// carry a relative slack of 2^-30 so that a change meeting one exactly (a
+// one-molecule change at the floor) holds whatever the rounding of the
+// products.  A negative or overflowing count breaks it too.  av / crit:
+// propensities and critical flags of x.
+constexpr double kLcSlack = 1.0 + 0x1p-30;
+template <int MS, int MR>
+__device__ __forceinline__ uint32_t leap_condition_broken(const NetView& n, int c, int nc, double eps, double theta,
+                                                          const uint32_t (&x)[MS], const long long (&nx)[MS],
+                                                          const double (&av)[MR],
+                                                          const uint32_t (&crit)[(MR + 31) / 32]) {
+    uint32_t xn[MS];
+    bool neg = false;
+#pragma unroll
+    for (int s = 0; s < MS; ++s) {
+        neg = neg || (s < n.S && (nx[s] < 0 || nx[s] > 0xFFFFFFFFll));
+        xn[s] = (uint32_t)nx[s];
+    }
+    if (neg) return 1u;
+    uint32_t broken = 0u;
+#pragma unroll
+    for (int j = 0; j < MR; ++j) {
+        if (j >= n.R) break;
+        const int nr = n.nrs[j];
+        if (is_crit<MR>(crit, j) || !(av[j] > 0.0) || nr == 0) continue;
+        double aj = n.k[c * n.R + j];
+        for (int i = 0; i < nr; ++i) {
+            const int sp = n.rs[j * 3 + i], r = n.rr[j * 3 + i];
+            uint32_t xs = 0;
+#pragma unroll
+            for (int q = 0; q < MS; ++q)
+                if (q == sp) xs = xn[q];
+            aj = aj * binom_d(xs, r);
+        }
+        const double D = __builtin_fabs(aj - av[j]);
+        const double t = (theta * av[j]) * kLcSlack;
+        bool over = D > t * eps;
+        for (int i = 0; i < nr; ++i) {
+            const int sp = n.rs[j * 3 + i], r = n.rr[j * 3 + i];
+            uint32_t xs = 0;
+#pragma unroll
+            for (int q = 0; q < MS; ++q)
+                if (q == sp) xs = x[q];
+            over = over && D * (double)xs > t * (double)r;
+        }
+        if (over) broken = 1u;
+    }
+#pragma unroll
+    for (int s = 0; s < MS; ++s) {
+        if (s >= n.S) break;
+        if (!hop_active(n.d[c * n.S + s], x[s], nc)) continue;
+        if (__builtin_fabs((double)xn[s] - (double)x[s]) > (theta * fmax(eps * (double)x[s], 1.0)) * kLcSlack)
+            broken = 1u;
+    }
+    return broken;
 }
 
 // ---------------------------------------------------------------------------
@@ -777,7 +816,7 @@ __device__ __forceinline__ bool leap_subvolume(const KernelArgs& a, const NetVie
         Xn[ri.x_off + (uint64_t)s * N + v] = (uint32_t)nx[s];
     }
     fire_out = fire;
-    viol = leap_violation<MS>(n.hor, n.mult, n.S, x, nx, a.epsilon, a.theta);
+    viol = a.judge_on ? leap_condition_broken<MS, MR>(n, c, nc, a.epsilon, a.theta, x, nx, av, crit) : 0u;
     return bad;
 }
 
@@ -1140,38 +1179,49 @@ __device__ __forceinline__ bool leap_tile_fwd(const KernelArgs& a, const NetView
 
 // The end of the leap pass for one tile: new counts = counts + global
 // balances (which are cleared again), the critical event, the negativity
-// and post-leap checks, and the tile's statistics.
-template <int MS>
+// check, the post-leap leap condition (when the judge is on) and the tile's
+// statistics.  nx returns the lane's new counts (any sign).
+template <int MS, int MR>
 __device__ __forceinline__ void finish_tile(const KernelArgs& a, const NetView& n, const RealInfo& ri,
                                             const uint32_t* X, uint32_t* Xn, uint32_t v0, const StepView& sv,
-                                            unsigned long long& fire, uint32_t& bad, uint32_t& viol) {
+                                            unsigned long long& fire, uint32_t& bad, uint32_t& viol,
+                                            uint32_t (&x)[MS], long long (&nx)[MS]) {
     const uint32_t v = v0 + (threadIdx.x & 63u);
+#pragma unroll
+    for (int s = 0; s < MS; ++s) {
+        x[s] = 0u;
+        nx[s] = 0;
+    }
     if (v >= ri.N) return;
-    // one species at a time, few registers live
     const bool at_star = sv.crit && v == sv.vstar && (sv.ev_kind == 1 || sv.ev_kind == 2);
     const bool at_dst = sv.crit && sv.ev_kind == 2 && v == sv.ev_dst && sv.ev_dst != sv.vstar;
     if (at_star) fire += 1ull;
     bool neg = false;
-    uint32_t vio = 0u;
 #pragma unroll
     for (int s = 0; s < MS; ++s) {
         if (s >= n.S) break;
         const uint64_t at = ri.x_off + (uint64_t)s * ri.N + v;
-        const uint32_t x = X[at];
+        x[s] = X[at];
         const unsigned long long d = a.gdelta[at];
         if (d != 0ull) a.gdelta[at] = 0ull;
-        long long nx = (long long)x + (long long)d;
+        long long y = (long long)x[s] + (long long)d;
         if (at_star) {
-            if (sv.ev_kind == 1) nx += n.nu[s * n.R + sv.ev_idx];
-            else if (s == sv.ev_idx) nx -= 1;
+            if (sv.ev_kind == 1) y += n.nu[s * n.R + sv.ev_idx];
+            else if (s == sv.ev_idx) y -= 1;
         }
-        if (at_dst && s == sv.ev_idx) nx += 1;
-        neg = neg || nx < 0 || nx > 0xFFFFFFFFll;
-        Xn[at] = (uint32_t)nx;
-        if (species_violates(n.hor[s], n.mult[s], x, nx, a.epsilon, a.theta)) vio = 1u;
+        if (at_dst && s == sv.ev_idx) y += 1;
+        neg = neg || y < 0 || y > 0xFFFFFFFFll;
+        Xn[at] = (uint32_t)y;
+        nx[s] = y;
     }
     bad = neg ? 1u : 0u;
-    viol = vio;
+    if (a.judge_on) {
+        const int c = n.C > 1 ? a.comp[ri.comp_off + v] : 0;
+        double av[MR];
+        uint32_t crit[(MR + 31) / 32];
+        react_props<MR, MS>(n, x, c, a.n_critical, av, crit);
+        viol = leap_condition_broken<MS, MR>(n, c, a.n_critical, a.epsilon, a.theta, x, nx, av, crit);
+    }
 }
 
 __device__ __forceinline__ void reduce_partB(unsigned long long& fire, uint32_t& bad, uint32_t& viol) {
@@ -1215,9 +1265,13 @@ __device__ void finalize_B(const KernelArgs& a, const RealInfo& ri, uint32_t r, 
             decide(c);
             c.retry = 1u;
         }
+        c.prop_ok = 0u;
         a.ctl[r] = c;
         return;
     }
+    // the leap pass of a learned context proposed the next step's tau from
+    // the state it produced (edge-parallel pass only)
+    c.prop_ok = a.props_in_finish ? 1u : 0u;
     c.cur ^= 1u;
     c.t = c.last ? c.t_end : c.t + c.tau;
     c.step += 1u;
@@ -1350,7 +1404,9 @@ __global__ __launch_bounds__(256) void phase_b_finish_kernel(KernelArgs a) {
         const uint32_t cur = wave_uniform(cp->cur);
         unsigned long long fire = 0;
         uint32_t bad = 0, viol = 0;
-        finish_tile<MS>(a, n, ri, cur ? a.x1 : a.x0, cur ? a.x0 : a.x1, te.v0, sv, fire, bad, viol);
+        uint32_t x[MS];
+        long long nx[MS];
+        finish_tile<MS, MR>(a, n, ri, cur ? a.x1 : a.x0, cur ? a.x0 : a.x1, te.v0, sv, fire, bad, viol, x, nx);
         reduce_partB(fire, bad, viol);
         if (lane == 0) a.partB_tile[t] = PartB{a.partB_tile[t].fire + fire, bad, viol};
     }

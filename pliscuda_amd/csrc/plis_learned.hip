@@ -1,33 +1,49 @@
-// plis_learned.hip -- learned tau pass and training records.
+// plis_learned.hip -- learned tau proposal: fused into the leap pass,
+// standalone pass, and training records.
 //
-// learned_tau_kernel replaces the Cao part of the tau pass: per subvolume it
-// builds 12 local features (no neighbour gathers), runs the MLP on the
-// wavefront's 64-subvolume tile with bf16 MFMA, and proposes
-// tau_v = kappa * exp(log tau prediction).  The critical part stays exact
-// (critical propensity and the tau'' clock, as in the plain pass), so the
-// critical event is drawn exactly as in the plain method.  The leap pass
-// then judges the proposal (finalize_B).
+// The network replaces Cao's tau' selection: per subvolume it maps 12 local
+// features (no neighbour gathers) to a log-tau proposal and a critical
+// logit, on the wavefront's 64-subvolume tile with bf16 MFMA.  The
+// proposal is tau_v = kappa exp(log tau), halved where the critical logit
+// flags the subvolume (the network expects a leap at its proposal to break
+// the leap condition there); the step takes tau = min_v tau_v.  The
+// critical part stays exact (critical propensity, the tau'' clock and the
+// critical event, as in the plain pass).
 //
-// records_kernel writes (features, log Cao tau', critical flag) of every
-// subvolume of one realization: the training records.
+// learned_finish_kernel is the end of the edge-parallel leap pass with the
+// proposal fused in: once a tile's new counts are formed it computes their
+// features, runs the network and writes the tile partial the next step's
+// decide kernel reads, so a learned step has no tau pass at all.  The
+// partials are valid only if the step is accepted (commit sets prop_ok);
+// after a rejection, a leap-back or a change of state made by the host,
+// learned_tau_kernel recomputes them from the current state.  The leap pass
+// then judges the step (finalize_B).
+//
+// records_kernel writes training records of one realization's state:
+// features, log Cao tau'_v and the accept/reject label of a trial leap of v
+// at tau'_v (the post-leap leap condition at v).
 #pragma once
 #include "plis_kernels.hip"
 #include "plis_mlp.hip"
 
 namespace plis {
 
-// Features of the lane's subvolume (the host restatement computes the same
-// quantities in double; single-precision hardware logs are enough here):
+// Philox purpose of the records' trial leaps (the step's purposes are 1..5).
+constexpr uint32_t kTrial = 7u;
+
+// Features of one subvolume from its counts (the host restatement computes
+// the same quantities in double; single-precision hardware logs are enough
+// for a bf16 network):
 //   f[s] = log(1 + x_s), f[5+s] = log(1 + sum_{j: r_sj>0} a_j + d_s x_s W),
 //   f[10] = highest order among reactions with a_j > 0, f[11] = log prev tau.
+// a0c_out: the exact critical propensity (FP64, as the plain tau pass);
+// live: the subvolume has a non-critical reaction with a_j > 0 or an active
+// hop -- without one its proposal is infinite, so a realization with no
+// possible event stops as in the plain method.
 template <int MS, int MR>
 __device__ __forceinline__ void features_and_crit(const KernelArgs& a, const NetView& n, const RealInfo& ri,
-                                                  const uint32_t* X, uint32_t v, float prev_logtau, float (&f)[12],
-                                                  double& a0c_out) {
-    const uint32_t N = ri.N;
-    uint32_t x[MS];
-#pragma unroll
-    for (int s = 0; s < MS; ++s) x[s] = s < n.S ? X[ri.x_off + (uint64_t)s * N + v] : 0u;
+                                                  const uint32_t (&x)[MS], uint32_t v, float prev_logtau,
+                                                  float (&f)[12], double& a0c_out, bool& live) {
     const int c = n.C > 1 ? a.comp[ri.comp_off + v] : 0;
     double av[MR];
     uint32_t crit[(MR + 31) / 32];
@@ -36,11 +52,13 @@ __device__ __forceinline__ void features_and_crit(const KernelArgs& a, const Net
     double A[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     int maxo = 0;
     double a0c = 0.0;
+    bool lv = false;
 #pragma unroll
     for (int j = 0; j < MR; ++j) {
         if (j >= n.R) break;
         if (av[j] > 0.0 && n.order[j] > maxo) maxo = n.order[j];
         if (is_crit<MR>(crit, j)) a0c += av[j];
+        else if (av[j] > 0.0) lv = true;
         const int nr = n.nrs[j];
         for (int i = 0; i < nr; ++i) {
             const int s = n.rs[j * 3 + i];
@@ -54,6 +72,7 @@ __device__ __forceinline__ void features_and_crit(const KernelArgs& a, const Net
         if (s >= n.S) break;
         const double d = n.d[c * n.S + s];
         if (d > 0.0 && !hop_active(d, x[s], a.n_critical)) a0c += (d * (double)x[s]) * W;
+        if (hop_active(d, x[s], a.n_critical)) lv = true;
     }
 #pragma unroll
     for (int s = 0; s < 5; ++s) {
@@ -70,17 +89,58 @@ __device__ __forceinline__ void features_and_crit(const KernelArgs& a, const Net
     f[10] = (float)maxo;
     f[11] = prev_logtau;
     a0c_out = a0c;
+    live = lv;
 }
 
-template <int MS, int MR>
+// The proposals of one tile: features, the network on the tile, tau_v, the
+// exact tau'' clock of the given step, then min / argmin into the tile
+// partial.  Every lane of the wavefront takes part (the MLP is a wavefront
+// operation); act = the lane holds a subvolume.
+template <int MS, int MR, int ACT>
+__device__ __forceinline__ void propose_tile(const KernelArgs& a, const NetView& n, const RealInfo& ri,
+                                             MlpLdsInfer& S, const uint32_t (&x)[MS], bool act, uint32_t v,
+                                             uint32_t step, double prev_tau, uint32_t t) {
+    const MlpDev m{a.mlp_w32, (const __bf16*)a.mlp_q, a.mlp_norm};
+    const float prev_logtau = prev_tau > 0.0 ? __logf((float)prev_tau) : 0.0f;
+    float f[12];
+    double a0c = 0.0;
+    bool live = false;
+    if (act) {
+        features_and_crit<MS, MR>(a, n, ri, x, v, prev_logtau, f, a0c, live);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 12; ++k) f[k] = 0.0f;
+    }
+    stage_inputs(S.h, nullptr, m, f, act, kLdH);
+    float o0, o1;
+    mlp_infer_tile<ACT>(S, m, o0, o1);
+    double tp = kInf, t2 = kInf;
+    uint32_t v2 = v;
+    if (act) {
+        if (live) {
+            const float ly = o0 * a.mlp_norm[25] + a.mlp_norm[24];
+            tp = a.kappa * (double)__expf(ly);
+            if (o1 > 0.0f) tp = tp * 0.5;
+        }
+        if (a0c > 0.0) {
+            const U4 o = philox10(v, kSelfId, step, ctr_word3(ri.id, 0u, kTau2, 0u), a.seed_lo, a.seed_hi);
+            t2 = -dlog(uniform01(o.x)) / a0c;
+        }
+    }
+    reduce_partA(tp, t2, v2);
+    if ((threadIdx.x & 63u) == 0u) a.partA_tile[t] = PartA{tp, t2, v2, 0u};
+    wave_sync();
+}
+
+// Standalone learned tau pass: realizations in learned mode whose partials
+// do not hold proposals for the current state (first step, after a
+// rejection or leap-back, after the host changed the state or the network).
+template <int MS, int MR, int ACT>
 __global__ __launch_bounds__(256) void learned_tau_kernel(KernelArgs a) {
     __shared__ MlpLdsInfer s_mlp[4];
     const NetView n = net_view(a);
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wv = wave_uniform(threadIdx.x >> 6);
-    MlpLdsInfer& S = s_mlp[wv];
-    const MlpDev m{a.mlp_w32, (const __bf16*)a.mlp_q, a.mlp_norm};
-    const float y_mu = a.mlp_norm[24], y_sig = a.mlp_norm[25];
     for (uint32_t it = 0;; ++it) {
         const uint32_t t = tile_at(it, a.ntiles, a.waves_per_block);
         if (t == 0xFFFFFFFFu) break;
@@ -88,43 +148,66 @@ __global__ __launch_bounds__(256) void learned_tau_kernel(KernelArgs a) {
         const uint32_t r = wave_uniform(te.r);
         const RealInfo ri = ld_ro(a.info + r);
         const cptr<RealCtl> cp = ro(a.ctl + r);
-        if (wave_uniform(cp->done) || wave_uniform(cp->retry) || wave_uniform(cp->fallback)) continue;
+        if (wave_uniform(cp->done) || wave_uniform(cp->retry) || wave_uniform(cp->fallback) ||
+            wave_uniform(cp->prop_ok))
+            continue;
         const uint32_t* X = wave_uniform(cp->cur) ? a.x1 : a.x0;
-        const double prev = cp->tau;
-        const float prev_logtau = prev > 0.0 ? __logf((float)prev) : 0.0f;
         const uint32_t v = te.v0 + lane;
         const bool act = v < ri.N;
-        float f[12];
-        double a0c = 0.0;
-        if (act) {
-            features_and_crit<MS, MR>(a, n, ri, X, v, prev_logtau, f, a0c);
-        } else {
+        uint32_t x[MS];
 #pragma unroll
-            for (int k = 0; k < 12; ++k) f[k] = 0.0f;
-        }
-        stage_inputs(S.h, nullptr, m, f, act, kLdH);
-        float o0, o1;
-        mlp_infer_tile(S, m, o0, o1);
-        double tp = kInf, t2 = kInf;
-        uint32_t v2 = v;
-        if (act) {
-            tp = a.kappa * dexp((double)(o0 * y_sig + y_mu));
-            if (a0c > 0.0) {
-                const uint32_t step = wave_uniform(cp->step);
-                const U4 o = philox10(v, kSelfId, step, ctr_word3(ri.id, 0u, kTau2, 0u), a.seed_lo, a.seed_hi);
-                t2 = -dlog(uniform01(o.x)) / a0c;
-            }
-        }
-        reduce_partA(tp, t2, v2);
-        if (lane == 0) a.partA_tile[t] = PartA{tp, t2, v2, 0u};
-        wave_sync();
+        for (int s = 0; s < MS; ++s) x[s] = (act && s < n.S) ? X[ri.x_off + (uint64_t)s * ri.N + v] : 0u;
+        propose_tile<MS, MR, ACT>(a, n, ri, s_mlp[wv], x, act, v, wave_uniform(cp->step), cp->tau, t);
     }
 }
 
+// End of the edge-parallel leap pass with the next step's proposals fused
+// in (learned contexts): finish_tile, then propose_tile on the new counts
+// for step + 1.  Proposals of a realization whose step is rejected are
+// never read (commit clears prop_ok).
+template <int MS, int MR, bool MC, int ACT>
+__global__ __launch_bounds__(256) void learned_finish_kernel(KernelArgs a) {
+    __shared__ MlpLdsInfer s_mlp[4];
+    const NetView n = net_view<MC>(a);
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wv = wave_uniform(threadIdx.x >> 6);
+    for (uint32_t it = 0;; ++it) {
+        const uint32_t t = tile_at(it, a.ntiles, a.waves_per_block);
+        if (t == 0xFFFFFFFFu) break;
+        const TileEnt te = ld_ro(a.tiles + t);
+        const uint32_t r = wave_uniform(te.r);
+        const RealInfo ri = ld_ro(a.info + r);
+        const cptr<RealCtl> cp = ro(a.ctl + r);
+        if (wave_uniform(cp->done)) continue;
+        const StepView sv = step_view(ri, cp);
+        const uint32_t cur = wave_uniform(cp->cur);
+        unsigned long long fire = 0;
+        uint32_t bad = 0, viol = 0;
+        uint32_t x[MS];
+        long long nx[MS];
+        finish_tile<MS, MR>(a, n, ri, cur ? a.x1 : a.x0, cur ? a.x0 : a.x1, te.v0, sv, fire, bad, viol, x, nx);
+        reduce_partB(fire, bad, viol);
+        if (lane == 0) a.partB_tile[t] = PartB{a.partB_tile[t].fire + fire, bad, viol};
+        const uint32_t v = te.v0 + lane;
+        uint32_t xn[MS];
+#pragma unroll
+        for (int s = 0; s < MS; ++s) xn[s] = (uint32_t)(nx[s] < 0 ? 0 : nx[s]);
+        propose_tile<MS, MR, ACT>(a, n, ri, s_mlp[wv], xn, v < ri.N, v, sv.step + 1u, sv.tau, t);
+    }
+}
+
+// Training records of realization rec_real's current state: features,
+// log Cao tau'_v (capped at 1e6) and the reject label of a trial leap of v
+// alone at tau'_v (non-critical reactions, out-hops and in-hops at the
+// neighbours' propensities, drawn from the TRIAL streams with the complete
+// sampler: counter (v, kSelfId or u, step, purpose kTrial), block 96 + j/4
+// for reactions and 64 + s/2 for hops, word 2 (s & 1) + 0 out / 1 in);
+// label 1 where the leap condition breaks at v.
 template <int MS, int MR>
 __global__ __launch_bounds__(256) void records_kernel(KernelArgs a) {
     const NetView n = net_view(a);
     const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t k0 = a.seed_lo, k1 = a.seed_hi;
     for (uint32_t it = 0;; ++it) {
         const uint32_t t = tile_at(it, a.ntiles, a.waves_per_block);
         if (t == 0xFFFFFFFFu) break;
@@ -135,21 +218,89 @@ __global__ __launch_bounds__(256) void records_kernel(KernelArgs a) {
         const uint32_t* X = cp->cur ? a.x1 : a.x0;
         const double prev = cp->tau;
         const float prev_logtau = prev > 0.0 ? __logf((float)prev) : 0.0f;
+        const uint32_t step = cp->step;
         const uint32_t v = te.v0 + lane;
         if (v >= ri.N) continue;
+        const uint32_t N = ri.N;
+        uint32_t x[MS];
+#pragma unroll
+        for (int s = 0; s < MS; ++s) x[s] = s < n.S ? X[ri.x_off + (uint64_t)s * N + v] : 0u;
         float f[12];
         double a0c, tp, a0c2;
-        features_and_crit<MS, MR>(a, n, ri, X, v, prev_logtau, f, a0c);
+        bool live;
+        features_and_crit<MS, MR>(a, n, ri, x, v, prev_logtau, f, a0c, live);
         tau_subvolume<MS, MR>(a, n, ri, X, v, tp, a0c2);
 #pragma unroll
         for (int k = 0; k < 12; ++k) a.rec_feats[(size_t)v * 12 + k] = f[k];
         a.rec_ylog[v] = tp < 1e6 ? (float)dlog(tp) : (float)dlog(1e6);
-        a.rec_crit[v] = a0c > 0.0 ? 1.0f : 0.0f;
+        float label = 0.0f;
+        if (tp < kInf) {
+            const int c = n.C > 1 ? a.comp[ri.comp_off + v] : 0;
+            double av[MR];
+            uint32_t crit[(MR + 31) / 32];
+            react_props<MR, MS>(n, x, c, a.n_critical, av, crit);
+            long long nx[MS];
+#pragma unroll
+            for (int s = 0; s < MS; ++s) nx[s] = (long long)x[s];
+            const uint32_t c3 = ctr_word3(ri.id, 0u, kTrial, 0u);
+            for (int j = 0; j < n.R; ++j) {
+                if (is_crit<MR>(crit, j) || !(av[j] > 0.0)) continue;
+                const uint32_t cb = c3 | (96u + ((uint32_t)j >> 2)), wd = (uint32_t)j & 3u;
+                const U4 o = philox10(v, kSelfId, step, cb, k0, k1);
+                const long long k = poisson(tp * av[j], pick(o, wd), DrawId{v, kSelfId, step, cb, k0, k1, wd});
+#pragma unroll
+                for (int s = 0; s < MS; ++s)
+                    if (s < n.S) nx[s] += k * (long long)n.nu[s * n.R + j];
+            }
+            const bool weighted = (ri.flags & 2u) != 0;
+            const bool sym = (ri.flags & 1u) != 0;
+            const uint32_t rb = a.rp[ri.rp_off + v], re = a.rp[ri.rp_off + v + 1];
+#pragma unroll
+            for (int s = 0; s < MS; ++s) {
+                if (s >= n.S) break;
+                const double d = n.d[c * n.S + s];
+                if (!hop_active(d, x[s], a.n_critical)) continue;
+                const double dx = d * (double)x[s];
+                const uint32_t cb = c3 | (64u + ((uint32_t)s >> 1)), wd = 2u * ((uint32_t)s & 1u);
+                for (uint32_t q = rb; q < re; ++q) {
+                    const uint32_t u = a.col[ri.e_off + q];
+                    const double wq = weighted ? (double)a.w[ri.e_off + q] : 1.0;
+                    const U4 o = philox10(v, u, step, cb, k0, k1);
+                    nx[s] -= poisson(tp * (dx * wq), pick(o, wd), DrawId{v, u, step, cb, k0, k1, wd});
+                }
+            }
+            const uint32_t* icol = sym ? a.col + ri.e_off : a.icol + ri.ie_off;
+            const float* iwp = sym ? a.w + ri.e_off : a.iw + ri.ie_off;
+            const uint32_t* irp = sym ? a.rp + ri.rp_off : a.irp + ri.irp_off;
+            for (uint32_t q = irp[v]; q < irp[v + 1]; ++q) {
+                const uint32_t u = icol[q];
+                const double wq = weighted ? (double)iwp[q] : 1.0;
+                const int cu = n.C > 1 ? a.comp[ri.comp_off + u] : 0;
+#pragma unroll
+                for (int s = 0; s < MS; ++s) {
+                    if (s >= n.S) break;
+                    const uint32_t xu = X[ri.x_off + (uint64_t)s * N + u];
+                    const double du = n.d[cu * n.S + s];
+                    if (!hop_active(du, xu, a.n_critical)) continue;
+                    const uint32_t cb = c3 | (64u + ((uint32_t)s >> 1)), wd = 2u * ((uint32_t)s & 1u) + 1u;
+                    const U4 o = philox10(v, u, step, cb, k0, k1);
+                    nx[s] += poisson(tp * ((du * (double)xu) * wq), pick(o, wd), DrawId{v, u, step, cb, k0, k1, wd});
+                }
+            }
+            label = leap_condition_broken<MS, MR>(n, c, a.n_critical, a.epsilon, a.theta, x, nx, av, crit) ? 1.0f
+                                                                                                            : 0.0f;
+        }
+        a.rec_crit[v] = label;
     }
 }
 
-#define PLIS_INST_L(MS, MR)                                                  \
-    template __global__ void learned_tau_kernel<MS, MR>(KernelArgs);        \
+#define PLIS_INST_L(MS, MR)                                                          \
+    template __global__ void learned_tau_kernel<MS, MR, 0>(KernelArgs);             \
+    template __global__ void learned_tau_kernel<MS, MR, 1>(KernelArgs);             \
+    template __global__ void learned_finish_kernel<MS, MR, false, 0>(KernelArgs);   \
+    template __global__ void learned_finish_kernel<MS, MR, false, 1>(KernelArgs);   \
+    template __global__ void learned_finish_kernel<MS, MR, true, 0>(KernelArgs);    \
+    template __global__ void learned_finish_kernel<MS, MR, true, 1>(KernelArgs);    \
     template __global__ void records_kernel<MS, MR>(KernelArgs);
 PLIS_INST_L(1, 4)
 PLIS_INST_L(3, 4)

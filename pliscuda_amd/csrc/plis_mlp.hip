@@ -1,6 +1,6 @@
 // plis_mlp.hip -- the learned tau proposal on gfx950 matrix cores.
 //
-// MLP 12 -> 64 -> 64 -> 2 (ReLU), bfloat16 operands with fp32
+// MLP 12 -> 64 -> 64 -> 2 (GELU or ReLU), bfloat16 operands with fp32
 // accumulation on v_mfma_f32_32x32x16_bf16.  One wavefront owns a tile of 64
 // rows (subvolumes or training samples); activations are staged in LDS in
 // both row- and column-major form so every MFMA fragment is a contiguous
@@ -41,7 +41,38 @@ struct MlpDev {
     const float* norm;       // fmu[12] fsig[12] y_mu y_sig
 };
 
-__device__ __forceinline__ float act_f(float z) { return fmaxf(z, 0.0f); }
+// Activations: ACT 0 = ReLU, 1 = GELU.  GELU(z) = z Phi(z) is evaluated as
+//     max(z, 0) - |z| max(0, p(|z|)),  p(a) = 0.5 + a (C1 + a (C2 + a C3)),
+// a cubic fitted minimax over the real line to the correction
+// |z| Phi(-|z|) = relu(z) - GELU(z), with p(0) = 1/2 so that the slope at 0
+// is GELU's 1/2: largest error 0.0058 against the erf form, 7 VALU
+// instructions, no transcendental.
+constexpr float kGeluC1 = -0.47735822f, kGeluC2 = 0.15120307f, kGeluC3 = -0.01585033f;
+
+template <int ACT>
+__device__ __forceinline__ float act_f(float z) {
+    if constexpr (ACT == 0) {
+        return fmaxf(z, 0.0f);
+    } else {
+        const float a = __builtin_fabsf(z);
+        const float p = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(kGeluC3, a, kGeluC2), a, kGeluC1), a, 0.5f);
+        return fmaxf(z, 0.0f) - a * fmaxf(p, 0.0f);
+    }
+}
+
+// d act / dz (the backward pass keeps it from the forward pass)
+template <int ACT>
+__device__ __forceinline__ float act_d(float z) {
+    if constexpr (ACT == 0) {
+        return z > 0.0f ? 1.0f : 0.0f;
+    } else {
+        const float a = __builtin_fabsf(z);
+        const float p = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(kGeluC3, a, kGeluC2), a, kGeluC1), a, 0.5f);
+        const float dp = __builtin_fmaf(__builtin_fmaf(3.0f * kGeluC3, a, 2.0f * kGeluC2), a, kGeluC1);
+        const float q = p > 0.0f ? __builtin_fmaf(a, dp, p) : 0.0f;
+        return z > 0.0f ? 1.0f - q : q;
+    }
+}
 
 // lane-layout of v_mfma_f32_32x32x16_bf16: A fragment = 8 consecutive k of
 // row (l % 32) starting at k = 8*(l/32); B fragment = 8 consecutive k of
@@ -67,6 +98,7 @@ struct MlpLds {
     __bf16 h1[64 * kLdH], h1t[64 * kLdH];
     __bf16 h2[64 * kLdH], h2t[64 * kLdH];
     __bf16 dz[64 * kLdH], dzt[64 * kLdH];
+    __bf16 g1[64 * kLdH], g2[64 * kLdH];  // act'(z) of the hidden layers
     float out[64 * 2];
 };
 
@@ -98,10 +130,11 @@ __device__ __forceinline__ void stage_inputs(__bf16* x, __bf16* xt, const MlpDev
 }
 
 // One dense layer on the 64-row tile: out[64][N] = act(in[64][K] . W + b).
-// Writes bf16 activations (row-major and transposed).
-template <int K, int NB, bool ACT>
+// Writes bf16 activations (row-major and transposed) and, for the hidden
+// layers, act'(z) in bf16 (g) for the backward pass.
+template <int K, int NB, bool ACTON, int ACT>
 __device__ __forceinline__ void dense(const __bf16* in, int ldin, const __bf16* wt, const float* bias,
-                                      __bf16* h, __bf16* ht, float* out2) {
+                                      __bf16* h, __bf16* ht, __bf16* g, float* out2) {
 #pragma unroll
     for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
@@ -114,11 +147,12 @@ __device__ __forceinline__ void dense(const __bf16* in, int ldin, const __bf16* 
 #pragma unroll
             for (int i = 0; i < 16; ++i) {
                 const int row = mb * 32 + acc_row(i);
-                if (ACT) {
+                if (ACTON) {
                     const float z = c[i] + bias[col];
-                    const __bf16 a = (__bf16)act_f(z);
+                    const __bf16 a = (__bf16)act_f<ACT>(z);
                     h[row * kLdH + col] = a;
                     if (ht) ht[col * kLdH + row] = a;
+                    g[row * kLdH + col] = (__bf16)act_d<ACT>(z);
                 } else if (col < kNo) {
                     out2[row * 2 + col] = c[i] + bias[col];
                 }
@@ -126,22 +160,19 @@ __device__ __forceinline__ void dense(const __bf16* in, int ldin, const __bf16* 
         }
 }
 
-// Forward pass of the tile; returns the lane's two outputs.  Transposed
-// activations are kept only when h1t/h2t are given
-// (training).
-__device__ __forceinline__ void mlp_forward_tile(const __bf16* x, __bf16* h1, __bf16* h1t, __bf16* h2,
-                                                 __bf16* h2t, float* out,
-                                                 const MlpDev& m, float& o0, float& o1) {
+// Forward pass of the tile; returns the lane's two outputs.
+template <int ACT>
+__device__ __forceinline__ void mlp_forward_tile(MlpLds& s, const MlpDev& m, float& o0, float& o1) {
     wave_sync();
-    dense<16, 2, true>(x, kLdX, m.q + kQW1t, m.w32 + kPB1, h1, h1t, nullptr);
+    dense<16, 2, true, ACT>(s.x, kLdX, m.q + kQW1t, m.w32 + kPB1, s.h1, s.h1t, s.g1, nullptr);
     wave_sync();
-    dense<64, 2, true>(h1, kLdH, m.q + kQW2t, m.w32 + kPB2, h2, h2t, nullptr);
+    dense<64, 2, true, ACT>(s.h1, kLdH, m.q + kQW2t, m.w32 + kPB2, s.h2, s.h2t, s.g2, nullptr);
     wave_sync();
-    dense<64, 1, false>(h2, kLdH, m.q + kQW3t, m.w32 + kPB3, nullptr, nullptr, out);
+    dense<64, 1, false, ACT>(s.h2, kLdH, m.q + kQW3t, m.w32 + kPB3, nullptr, nullptr, nullptr, s.out);
     wave_sync();
     const int r = threadIdx.x & 63;
-    o0 = out[r * 2];
-    o1 = out[r * 2 + 1];
+    o0 = s.out[r * 2];
+    o1 = s.out[r * 2 + 1];
 }
 
 // ---------------------------------------------------------------------------
@@ -161,6 +192,7 @@ struct TrainArgs {
     float* partial;          // [gridDim.x][kPTotal + 1] (last = loss sum)
 };
 
+template <int ACT>
 __global__ __launch_bounds__(64) void mlp_train_kernel(TrainArgs a) {
     __shared__ MlpLds s;
     const int l = threadIdx.x & 63;
@@ -187,7 +219,7 @@ __global__ __launch_bounds__(64) void mlp_train_kernel(TrainArgs a) {
         for (int k = 0; k < 12; ++k) f[k] = a.feats[rec * 12 + k];
         stage_inputs(s.x, s.xt, a.m, f, true);
         float o0, o1;
-        mlp_forward_tile(s.x, s.h1, s.h1t, s.h2, s.h2t, s.out, a.m, o0, o1);
+        mlp_forward_tile<ACT>(s, a.m, o0, o1);
         const float y = a.ylog[rec], c = a.ycrit[rec];
         const float sp = fmaxf(o1, 0.f) + log1pf(expf(-fabsf(o1)));
         loss += (o0 - y) * (o0 - y) + sp - c * o1;
@@ -228,8 +260,7 @@ __global__ __launch_bounds__(64) void mlp_train_kernel(TrainArgs a) {
 #pragma unroll
                 for (int i = 0; i < 16; ++i) {
                     const int row = mb * 32 + acc_row(i), col = nb * 32 + acc_col();
-                    // relu'(z2) = 1 exactly where the stored activation is positive
-                    const __bf16 v = (__bf16)((float)s.h2[row * kLdH + col] > 0.0f ? dh[mb * 2 + nb][i] : 0.0f);
+                    const __bf16 v = (__bf16)(dh[mb * 2 + nb][i] * (float)s.g2[row * kLdH + col]);
                     s.dz[row * kLdH + col] = v;
                     s.dzt[col * kLdH + row] = v;
                 }
@@ -263,7 +294,7 @@ __global__ __launch_bounds__(64) void mlp_train_kernel(TrainArgs a) {
 #pragma unroll
                 for (int i = 0; i < 16; ++i) {
                     const int row = mb * 32 + acc_row(i), col = nb * 32 + acc_col();
-                    const __bf16 v = (__bf16)((float)s.h1[row * kLdH + col] > 0.0f ? dh[mb * 2 + nb][i] : 0.0f);
+                    const __bf16 v = (__bf16)(dh[mb * 2 + nb][i] * (float)s.g1[row * kLdH + col]);
                     s.dz[row * kLdH + col] = v;
                     s.dzt[col * kLdH + row] = v;
                 }
@@ -386,7 +417,7 @@ __global__ __launch_bounds__(256) void mlp_pack_kernel(const float* w32, __bf16*
 // into the row-major activations the next layer reads.  The two halves of
 // the tile (rows 0..31, 32..63) go one after the other; a half's B
 // fragments are read before its rows are written, so `out` may be `in`.
-template <int K>
+template <int K, int ACT>
 __device__ __forceinline__ void infer_layer(const __bf16* in, int ldin, const __bf16* wt, const float* bias,
                                             __bf16* out) {
     typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
@@ -408,10 +439,10 @@ __device__ __forceinline__ void infer_layer(const __bf16* in, int ldin, const __
                 const int u0 = mb * 32 + 8 * j + 4 * (l >> 5);
                 const float4 bb = *reinterpret_cast<const float4*>(bias + u0);
                 bf16x4 v;
-                v[0] = (__bf16)act_f(c[4 * j + 0] + bb.x);
-                v[1] = (__bf16)act_f(c[4 * j + 1] + bb.y);
-                v[2] = (__bf16)act_f(c[4 * j + 2] + bb.z);
-                v[3] = (__bf16)act_f(c[4 * j + 3] + bb.w);
+                v[0] = (__bf16)act_f<ACT>(c[4 * j + 0] + bb.x);
+                v[1] = (__bf16)act_f<ACT>(c[4 * j + 1] + bb.y);
+                v[2] = (__bf16)act_f<ACT>(c[4 * j + 2] + bb.z);
+                v[3] = (__bf16)act_f<ACT>(c[4 * j + 3] + bb.w);
                 *reinterpret_cast<bf16x4*>(out + row * kLdH + u0) = v;
             }
         }
@@ -422,11 +453,12 @@ __device__ __forceinline__ void infer_layer(const __bf16* in, int ldin, const __
 // Forward pass of one 64-row tile for inference (inputs staged in s.h);
 // returns lane r's two outputs (row r).  The output layer is W3^T (padded
 // to 32 rows) times the tile: lanes 0..31 receive rows l and 32 + l.
+template <int ACT>
 __device__ __forceinline__ void mlp_infer_tile(MlpLdsInfer& s, const MlpDev& m, float& o0, float& o1) {
     wave_sync();
-    infer_layer<kNin>(s.h, kLdH, m.q + kQW1t, m.w32 + kPB1, s.h);
+    infer_layer<kNin, ACT>(s.h, kLdH, m.q + kQW1t, m.w32 + kPB1, s.h);
     wave_sync();
-    infer_layer<kNh>(s.h, kLdH, m.q + kQW2t, m.w32 + kPB2, s.h);
+    infer_layer<kNh, ACT>(s.h, kLdH, m.q + kQW2t, m.w32 + kPB2, s.h);
     wave_sync();
     const int l = threadIdx.x & 63;
     float a0 = 0.0f, a1 = 0.0f, b0 = 0.0f, b1 = 0.0f;
@@ -450,6 +482,7 @@ __device__ __forceinline__ void mlp_infer_tile(MlpLdsInfer& s, const MlpDev& m, 
 }
 
 // Inference over given feature rows (parity checks): out[n][2].
+template <int ACT>
 __global__ __launch_bounds__(64) void mlp_infer_kernel(MlpDev m, const float* feats, uint32_t n, float* out) {
     __shared__ MlpLdsInfer s;
     const int l = threadIdx.x & 63;
@@ -460,7 +493,7 @@ __global__ __launch_bounds__(64) void mlp_infer_kernel(MlpDev m, const float* fe
         for (int k = 0; k < 12; ++k) f[k] = r < n ? feats[(size_t)r * 12 + k] : 0.f;
         stage_inputs(s.h, nullptr, m, f, r < n, kLdH);
         float o0, o1;
-        mlp_infer_tile(s, m, o0, o1);
+        mlp_infer_tile<ACT>(s, m, o0, o1);
         if (r < n) {
             out[(size_t)r * 2] = o0;
             out[(size_t)r * 2 + 1] = o1;

@@ -63,6 +63,10 @@ struct RealCtl {
     // rejected proposal uses Cao's formula
     uint32_t learned, fallback;
     uint64_t learned_steps, learned_rejects, violations;
+    // the tile partials hold the network's proposals for the current state
+    // (written by the leap pass that produced it); cleared whenever the
+    // state or the network changes otherwise
+    uint32_t prop_ok, pad2;
 };
 
 struct TileEnt {
@@ -118,18 +122,22 @@ struct KernelArgs {
     double* dbg_a0c;
     double* dbg_t2;
     uint32_t waves_per_block;
-    // post-leap check |x' - x| <= theta max(eps x / g, 1) and learned tau
+    // post-leap leap condition (propensity form, tolerance theta) and
+    // learned tau; judge_on: evaluate the condition in the leap pass
     double theta;
     double viol_rate_max;    // accepted fraction of violating subvolumes (learned steps)
     double kappa;            // multiplier of the learned proposal
     int32_t learned_on;
+    int32_t judge_on;
+    int32_t props_in_finish; // the leap pass proposes the next step's tau
+    int32_t mlp_act;         // activation of the network: 0 ReLU, 1 GELU
     const float* mlp_w32;    // fp32 master blob
     const void* mlp_q;       // packed bf16 copies
     const float* mlp_norm;   // feature / target standardisation
     // records output (records kernel)
     float* rec_feats;
     float* rec_ylog;
-    float* rec_crit;
+    float* rec_crit;         // trial-leap reject label
     uint32_t rec_real;
 };
 

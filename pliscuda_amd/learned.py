@@ -4,11 +4,15 @@ training of the 12-64-64-2 network and the learned stepping mode.
 The network maps 12 local features of a subvolume (log1p counts of the first
 five species, log1p of their consuming propensities, highest active reaction
 order, log of the previous global tau) to a standardised log tau proposal and
-a critical-flag logit.  Training data are (features, log Cao tau', critical
-flag) records of plain runs; the learned step uses
-tau = kappa * min_v exp(proposal_v) and is judged after the leap by Cao's own
-leap condition (no negative population and a violation rate no higher than
-the plain method's), falling back to Cao's tau for that step otherwise.
+a critical-flag logit.  Training data are records the simulator generates in
+plain runs: (features, log Cao tau'_v, accept/reject of a trial leap of v at
+tau'_v under the post-leap leap condition).  The log-tau head regresses Cao's
+tau', the critical head the reject label.  The learned step uses
+tau = min_v tau_v, tau_v = kappa exp(proposal_v), halved where the critical
+logit is positive, with the proposals computed inside the previous leap pass;
+it is judged after the leap by the leap condition (no negative population and
+no more breaking subvolumes than the plain method's rate allows) and falls
+back to Cao's tau for that step otherwise.
 """
 from __future__ import annotations
 
@@ -53,9 +57,15 @@ def standardize_y(ylog, norm):
     return ((ylog - norm[24]) / norm[25]).astype(np.float32)
 
 
-def collect_records(ctx, snapshots, stride, per_snapshot, seed=0, t_end=np.inf):
-    """Run ctx in plain mode; every `stride` accepted steps take the records
-    of every realization and keep a random subset of `per_snapshot` rows."""
+THETA = 1.0  # tolerance of the post-leap leap condition (the judge and the labels)
+
+
+def collect_records(ctx, snapshots, stride, per_snapshot, seed=0, t_end=np.inf, theta=THETA):
+    """Run ctx in plain mode with the leap condition evaluated (its
+    statistics calibrate the judge); every `stride` accepted steps take the
+    records of every realization and keep a random subset of `per_snapshot`
+    rows.  Returns (features, log Cao tau', reject label)."""
+    ctx.set_tau_mode(False, theta=theta)
     rng = np.random.Generator(np.random.PCG64(seed))
     F, Y, Cc = [], [], []
     base = [ctx.stats(r)["steps"] for r in range(ctx.n)]
@@ -71,7 +81,8 @@ def collect_records(ctx, snapshots, stride, per_snapshot, seed=0, t_end=np.inf):
 
 
 def train(ctx, feats, ylog, crit, steps=2000, batch=65536, lr=1e-3, seed=1, w0=None):
-    """Standardise, set initial weights, train on the GPU; returns (norm, losses)."""
+    """Standardise, set initial weights, train on the GPU (log-tau head on
+    log Cao tau', critical head on the reject label); returns (norm, losses)."""
     norm = fit_norm(feats, ylog)
     ctx.mlp_set(init_weights(seed) if w0 is None else w0, norm)
     ys = standardize_y(ylog, norm)
@@ -80,8 +91,8 @@ def train(ctx, feats, ylog, crit, steps=2000, batch=65536, lr=1e-3, seed=1, w0=N
 
 
 def calibrate_viol_rate(stats_list, N, margin=2.0):
-    """Judge threshold from the plain method's own post-leap violations:
-    margin x the observed fraction of violating subvolumes per subvolume-step.
+    """Judge threshold from the plain method's own post-leap leap condition:
+    margin x the observed fraction of subvolumes breaking it per subvolume-step.
     N is the realizations' subvolume counts (a list, one per entry of
     stats_list) or one count shared by all."""
     Ns = list(N) if np.ndim(N) else [N] * len(stats_list)

@@ -91,7 +91,7 @@ def test_brusselator_mesh_1M():
     st = ffi.run(m, x, wl["eps"], wl["nc"], SEED, np.inf, step_limit=10)
     assert np.array_equal(ctx.get_state(), x), "state after 10 steps differs"
     _same_stats(ctx.stats(), st, "10 steps")
-    assert st.firings > 10 * 1_000_000
+    assert st.firings > 10 * 300_000
     ctx.close()
 
 

@@ -200,18 +200,20 @@ def test_paused_run_equals_straight_run():
     b.close()
 
 
-def _violations_match(net, topo, x0, eps, steps=15, seed=9):
-    """Every accepted step adds the subvolumes whose realized change broke
-    theta * max(eps x / g, 1) (theta = 3) to the statistics; step by step this
-    must be the oracle's count between the same two states."""
+def _violations_match(net, topo, x0, eps, steps=15, seed=9, theta=1.0):
+    """Every accepted step adds the subvolumes where the post-leap leap
+    condition (propensity form, tolerance theta) broke to the statistics;
+    step by step this must be the oracle's count between the same two
+    states."""
     ctx = capi.Context(net, [topo], [x0], eps=eps, seed=seed)
+    ctx.set_tau_mode(False, theta=theta)
     m = ffi.Model(net, topo)
     prev, before, total = np.array(x0, dtype=np.uint32), 0, 0
     for k in range(1, steps + 1):
         ctx.run(np.inf, step_limit=k)
         x = ctx.get_state()
         v = ctx.stats()["violations"]
-        n, _ = ffi.violations(m, prev, x, eps, 10, 3.0)
+        n, _ = ffi.violations(m, prev, x, eps, 10, theta)
         assert v - before == n, (k, v - before, n)
         total += n
         prev, before = x, v

@@ -1,11 +1,18 @@
-"""Pins the oracle's MLP: its gradient against central finite differences
-(double precision, no bfloat16 rounding) and Adam against the published
-update rule."""
+"""Pins the oracle's learned-tau functions: the MLP gradient against
+central finite differences (double precision, no bfloat16 rounding) for
+both activations, the GELU approximation against the erf form, Adam
+against the published update rule, and the features, the post-leap leap
+condition and the records' trial-leap labels against hand-computed values
+and closed forms."""
+import math
+
 import numpy as np
 import pytest
+from scipy import special, stats
 
 from oracle import ffi
 from pliscuda_amd import learned
+from pliscuda_amd import workloads as W
 
 
 def _data(n, seed):
@@ -17,11 +24,12 @@ def _data(n, seed):
     return f, y, c, norm
 
 
-def test_gradient_matches_finite_differences():
+@pytest.mark.parametrize("act", [0, 1])
+def test_gradient_matches_finite_differences(act):
     f, y, c, norm = _data(48, 1)
     w = learned.init_weights(3)
     w[1024:1088] = np.random.default_rng(5).normal(size=64) * 0.1  # nonzero b1
-    loss, g = ffi.mlp_loss_grad(w, norm, f, y, c, bf16=False)
+    loss, g = ffi.mlp_loss_grad(w, norm, f, y, c, bf16=False, act=act)
     rng = np.random.default_rng(7)
     idx = np.concatenate([rng.choice(np.arange(0, 12 * 64), 15), rng.choice(np.arange(1024, 5378), 35)])
     h = 1e-3
@@ -29,8 +37,8 @@ def test_gradient_matches_finite_differences():
         wp, wm = w.copy(), w.copy()
         wp[i] += h
         wm[i] -= h
-        lp, _ = ffi.mlp_loss_grad(wp, norm, f, y, c, bf16=False)
-        lm, _ = ffi.mlp_loss_grad(wm, norm, f, y, c, bf16=False)
+        lp, _ = ffi.mlp_loss_grad(wp, norm, f, y, c, bf16=False, act=act)
+        lm, _ = ffi.mlp_loss_grad(wm, norm, f, y, c, bf16=False, act=act)
         # the blob is float32: use the actually applied step
         fd = (lp - lm) / (float(wp[i]) - float(wm[i]))
         assert abs(fd - g[i]) <= 2e-3 * max(1.0, abs(g[i])) + 1e-5, (i, fd, g[i])
@@ -68,3 +76,155 @@ def test_calibrate_viol_rate_weights_realizations_by_size():
     assert one == pytest.approx(2 * 10 / (100 * 1000))
     # no observed violations: rate 0, the judge's Poisson bound still allows 2 per step
     assert learned.calibrate_viol_rate([{"violations": 0, "steps": 5}], 100) == 0.0
+
+
+def test_gelu_approximation_against_erf_form():
+    """max(z,0) - |z| max(0, p(|z|)) stays within 0.006 of z Phi(z) on the
+    whole line, equals relu beyond the cubic's root (|z| > 3.78), is exact at
+    0 and has GELU's slope 1/2 there."""
+    z = np.linspace(-12, 12, 48001).astype(np.float32)
+    g = ffi.gelu(z).astype(np.float64)
+    ex = z * 0.5 * (1 + special.erf(z / np.sqrt(2.0)))
+    assert np.abs(g - ex).max() < 0.006
+    far = np.abs(z) > 3.8
+    assert np.array_equal(g[far], np.maximum(z[far], 0).astype(np.float64))
+    assert ffi.gelu(np.float32(0.0)) == 0.0
+    h = 1e-3
+    slope = (float(ffi.gelu(np.float32(h))) - float(ffi.gelu(np.float32(-h)))) / (2 * h)
+    assert abs(slope - 0.5) < 1e-3
+    # the minimum near z = -0.75 (GELU's is -0.170)
+    assert abs(g.min() + 0.170) < 0.006
+
+
+# ---------------------------------------------------------------------------
+# features, post-leap leap condition, records
+# ---------------------------------------------------------------------------
+def _isolated(N):
+    return {"N": N, "row_ptr": np.zeros(N + 1, np.uint32), "col": np.zeros(0, np.uint32)}
+
+
+def _pair(w01=1.0, w10=1.0):
+    return {"N": 2, "row_ptr": np.array([0, 1, 2], np.uint32), "col": np.array([1, 0], np.uint32),
+            "w": np.array([w01, w10], np.float32)}
+
+
+def _viol(net, topo, x, xn, theta=1.0, eps=0.03, nc=10):
+    m = ffi.Model(net, topo)
+    n, flags = ffi.violations(m, np.array(x, np.uint32), np.array(xn, np.uint32), eps, nc, theta)
+    return n, flags
+
+
+def test_leap_condition_first_order_by_hand():
+    """A -> 0 at x = 100, eps = 0.03: a = k x changes by k |dx|, so the bound
+    theta max(eps, 1/x) a = 3k (theta = 1): |dx| = 3 holds, 4 breaks; theta =
+    2 allows 6."""
+    net = W.network(["A"], [({"A": 1}, {})], [0.7], [0.0])
+    topo = _isolated(1)
+    assert _viol(net, topo, [100], [97])[0] == 0
+    assert _viol(net, topo, [100], [103])[0] == 0
+    assert _viol(net, topo, [100], [96])[0] == 1
+    assert _viol(net, topo, [100], [104])[0] == 1
+    assert _viol(net, topo, [100], [94], theta=2.0)[0] == 0
+    assert _viol(net, topo, [100], [93], theta=2.0)[0] == 1
+    # at x = 20 the one-molecule floor 1/x = 0.05 > eps sets the bound: 1 molecule
+    assert _viol(net, topo, [20], [19])[0] == 0
+    assert _viol(net, topo, [20], [18])[0] == 1
+
+
+def test_leap_condition_second_order_by_hand():
+    """2A -> 0 at x = 20 (not critical: floor(20/2) = 10 = n_c): a = k x(x-1)/2
+    = 190 k; phi = r/x = 0.1 > eps, bound 0.1 a = 19 k.  x' = 19 gives
+    |da| = 19 k (holds, not strictly above), x' = 21 gives 20 k and x' = 18
+    gives 37 k (break)."""
+    net = W.network(["A"], [({"A": 2}, {})], [1e-3], [0.0])
+    topo = _isolated(1)
+    assert _viol(net, topo, [20], [19])[0] == 0
+    assert _viol(net, topo, [20], [21])[0] == 1
+    assert _viol(net, topo, [20], [18])[0] == 1
+
+
+def test_leap_condition_skips_critical_and_zero_order_channels():
+    """A critical channel (x = 5 < n_c) fires at most once exactly and is not
+    bounded; a zero-order source never changes its propensity."""
+    net = W.network(["A"], [({"A": 1}, {}), ({}, {"A": 1})], [0.7, 5.0], [0.0])
+    topo = _isolated(1)
+    assert _viol(net, topo, [5], [0])[0] == 0
+    assert _viol(net, topo, [5], [60])[0] == 0
+    # once A is above n_c the decay channel is bounded again
+    assert _viol(net, topo, [50], [60])[0] == 1
+
+
+def test_leap_condition_hop_channels_by_hand():
+    """A diffusing species with no reactions: its hop channel d x w changes by
+    the factor x'/x, bound theta max(eps x, 1): at x = 50, 1.5 molecules."""
+    net = W.network(["X"], [], np.zeros(0), [1.0])
+    topo = _pair()
+    n, f = _viol(net, topo, [50, 400], [49, 412])
+    assert n == 0 and not f.any()
+    n, f = _viol(net, topo, [50, 400], [52, 388])
+    assert list(f) == [1, 0]
+    n, f = _viol(net, topo, [50, 400], [51, 387])
+    assert list(f) == [0, 1]
+    # below n_c the hop is critical and not bounded
+    n, f = _viol(net, topo, [5, 400], [0, 405])
+    assert list(f) == [0, 0]
+
+
+def test_features_by_hand():
+    """Two subvolumes joined by an edge of weight 0.5 from 0 and 2 from 1;
+    A + B -> C (k 0.01), C -> 0 (k 0.1), diffusion d = (0.5, 0, 0.2)."""
+    net = W.network(["A", "B", "C"], [({"A": 1, "B": 1}, {"C": 1}), ({"C": 1}, {})], [0.01, 0.1],
+                    [0.5, 0.0, 0.2])
+    topo = _pair(0.5, 2.0)
+    x = np.array([30, 7, 12, 0, 40, 3], np.uint32)  # [s * 2 + v]
+    m = ffi.Model(net, topo)
+    f = ffi.features(m, x, 0.03, 10, prev_logtau=-4.5)
+    # subvolume 0: A = 30, B = 12, C = 40, W = 0.5
+    a1, a2 = 0.01 * 30 * 12, 0.1 * 40
+    want0 = [math.log1p(30), math.log1p(12), math.log1p(40), 0, 0,
+             math.log1p(a1 + 0.5 * 30 * 0.5), math.log1p(a1), math.log1p(a2 + 0.2 * 40 * 0.5), 0, 0,
+             2, -4.5]
+    # subvolume 1: A = 7, B = 0, C = 3, W = 2: A + B has no propensity, C -> 0 does
+    want1 = [math.log1p(7), 0.0, math.log1p(3), 0, 0,
+             math.log1p(0.5 * 7 * 2), 0.0, math.log1p(0.1 * 3 + 0.2 * 3 * 2), 0, 0, 1, -4.5]
+    np.testing.assert_allclose(f[0], want0, rtol=1e-6, atol=1e-6)
+    np.testing.assert_allclose(f[1], want1, rtol=1e-6, atol=1e-6)
+
+
+def test_records_label_rate_is_the_poisson_tail():
+    """20,000 isolated subvolumes with A -> 0 (k = 2) at x = 1000: Cao's
+    tau' = min(30 / (k x), 900 / (k x)) = 0.015, a trial leap fires
+    Poisson(30) molecules and breaks the condition iff more than 30 theta
+    leave; the label rate is P(Poisson(30) > 30 theta)."""
+    N = 20000
+    net = W.network(["A"], [({"A": 1}, {})], [2.0], [0.0])
+    m = ffi.Model(net, _isolated(N))
+    x = np.full(N, 1000, np.uint32)
+    for theta in (1.0, 1.5):
+        f, y, lab = ffi.records(m, x, 0.03, 10, 77, theta=theta)
+        assert np.all(y == np.float32(ffi.lib().orc_log(0.015)))
+        p = stats.poisson.sf(math.floor(30 * theta), 30)
+        assert abs(lab.mean() - p) < 3.5 * math.sqrt(p * (1 - p) / N), (theta, lab.mean(), p)
+    # another step index or realization draws other trial leaps
+    _, _, l1 = ffi.records(m, x, 0.03, 10, 77, step=1)
+    _, _, l0 = ffi.records(m, x, 0.03, 10, 77, step=0)
+    assert not np.array_equal(l0, l1)
+
+
+def test_records_label_counts_in_hops():
+    """A pure-diffusion pair with x = (2000, 0): subvolume 1 has no channel of
+    its own (tau' from in-hops only) and is labelled from the in-hops it
+    receives; subvolume 0 loses Poisson(d x w tau') molecules."""
+    N = 2
+    net = W.network(["X"], [], np.zeros(0), [1.0])
+    m = ffi.Model(net, _pair())
+    x = np.array([2000, 0], np.uint32)
+    tp, _, _ = ffi.tau_all(m, x, 0.03, 10, 5)
+    # v = 0: mu = -2000, sigma^2 = 2000, bound 60: tau' = min(0.03, 1.8) = 0.03
+    assert tp[0] == pytest.approx(0.03)
+    labs = np.array([ffi.records(m, x, 0.03, 10, 5, step=k)[2] for k in range(3000)])
+    # v = 0: Poisson(60) out, breaks iff more than 60 leave
+    p0 = stats.poisson.sf(60, 60)
+    assert abs(labs[:, 0].mean() - p0) < 3.5 * math.sqrt(p0 * (1 - p0) / 3000)
+    # v = 1 has x = 0: no hop channel of its own to bound, so it never breaks
+    assert labs[:, 1].sum() == 0
