@@ -187,6 +187,13 @@ int plis_mlp_get(plis_ctx *ctx, float *w32);
 int plis_mlp_activation(plis_ctx *ctx, int32_t act);
 /* outputs [n][2] (standardised log tau, critical logit) for feature rows [n][12] */
 int plis_mlp_forward(plis_ctx *ctx, const float *feats, uint32_t n, float *out);
+/* As plis_mlp_forward, but allocates nothing: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under
+ * graph capture, serving loops, harnesses that forbid it).  The rows and
+ * outputs are staged through a fixed per-GPU scratch array of the library
+ * (held until the process ends), so n is at most 16384 (PLIS_ERR_ARG
+ * beyond); calls that overlap on one GPU race on that scratch. */
+int plis_mlp_forward_noalloc(plis_ctx *ctx, const float *feats, uint32_t n, float *out);
 /* mean loss and its gradient over n (multiple of 64) records, no update;
  * loss per record = (o0 - ylog)^2 + softplus(o1) - ycrit o1 */
 int plis_mlp_loss_grad(plis_ctx *ctx, const float *feats, const float *ylog, const float *ycrit,

@@ -24,7 +24,7 @@ EXPORTS = ("plis_create", "plis_run", "plis_run_until", "plis_arm", "plis_iterat
            "plis_launch_shape", "plis_redo_tiles", "plis_checkpoint_size", "plis_checkpoint", "plis_restore", "plis_autotune", "plis_destroy", "plis_last_error",
            "plis_device_count", "plis_mlp_set", "plis_mlp_get", "plis_mlp_forward",
            "plis_mlp_loss_grad", "plis_mlp_train", "plis_records", "plis_set_tau_mode",
-           "plis_mlp_activation")
+           "plis_mlp_activation", "plis_mlp_forward_noalloc")
 MLP_PARAMS = 5378
 
 
@@ -101,6 +101,7 @@ def _declare(L):
     L.plis_mlp_set.argtypes = [vp, fp, fp]
     L.plis_mlp_get.argtypes = [vp, fp]
     L.plis_mlp_forward.argtypes = [vp, fp, C.c_uint32, fp]
+    L.plis_mlp_forward_noalloc.argtypes = [vp, fp, C.c_uint32, fp]
     L.plis_mlp_loss_grad.argtypes = [vp, fp, fp, fp, C.c_uint32, fp, P(C.c_double)]
     L.plis_mlp_train.argtypes = [vp, fp, fp, fp, C.c_uint64, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64, fp]
     L.plis_records.argtypes = [vp, C.c_int32, fp, fp, fp]
@@ -272,6 +273,15 @@ class Context:
         out = np.empty((f.shape[0], 2), dtype=np.float32)
         self._chk(lib().plis_mlp_forward(self.h, _ptr(f, C.c_float), f.shape[0], _ptr(out, C.c_float)),
                   "plis_mlp_forward")
+        return out
+
+    def mlp_forward_noalloc(self, feats):
+        """plis_mlp_forward through the library's fixed scratch (no device
+        allocation; at most 16384 rows)."""
+        f = np.ascontiguousarray(feats, dtype=np.float32)
+        out = np.empty((f.shape[0], 2), dtype=np.float32)
+        self._chk(lib().plis_mlp_forward_noalloc(self.h, _ptr(f, C.c_float), f.shape[0], _ptr(out, C.c_float)),
+                  "plis_mlp_forward_noalloc")
         return out
 
     def mlp_loss_grad(self, feats, ylog, ycrit):

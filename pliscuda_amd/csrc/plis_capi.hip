@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -21,6 +22,38 @@
 using namespace plis;
 
 static thread_local std::string g_create_err;
+
+// Scratch of plis_mlp_forward_noalloc: device staging for the feature rows
+// and the outputs of one call, [rows][12] then [rows][2] floats.  One array
+// per GPU, held until the process ends; every call on a GPU shares it.  In
+// host code `scratch` names a host-side stand-in: kernels and copies use the
+// device address taken by hipGetSymbolAddress in plis_create instead.
+constexpr uint32_t kNoAllocRows = 16384;
+constexpr size_t kNoAllocFloats = (size_t)kNoAllocRows * 14;
+static __device__ __align__(256) float scratch[kNoAllocFloats];
+constexpr int kMaxDevices = 64;
+static float* g_scratch_ptr[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
+
+// Device address of this device's scratch, taken once per device with the
+// device current (plis_create calls it).  nullptr when the lookup fails.
+static float* scratch_ptr(int device, std::string& err) {
+    if (device < 0 || device >= kMaxDevices) {
+        err = "device index out of range for the no-allocation scratch";
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return nullptr;
+        }
+        g_scratch_ptr[device] = static_cast<float*>(p);
+    }
+    return g_scratch_ptr[device];
+}
 
 constexpr int kGraphSizes = 7;
 
@@ -620,6 +653,14 @@ extern "C" int plis_create(const plis_network* net, int32_t n_real, const plis_t
 
     if (hipSetDevice(0) != hipSuccess) { delete c; g_create_err = "hipSetDevice failed"; return PLIS_ERR_NODEV; }
     c->device = 0;
+    {
+        std::string serr;
+        if (!scratch_ptr(c->device, serr)) {
+            delete c;
+            g_create_err = serr;
+            return PLIS_ERR_HIP;
+        }
+    }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, 0) != hipSuccess) { delete c; g_create_err = "hipGetDeviceProperties failed"; return PLIS_ERR_HIP; }
     c->cus = prop.multiProcessorCount;
@@ -1139,6 +1180,42 @@ extern "C" int plis_mlp_forward(plis_ctx* c, const float* feats, uint32_t n, flo
         c->err = std::string("plis_mlp_forward: ") + hipGetErrorString(e);
         return PLIS_ERR_HIP;
     }
+    return PLIS_OK;
+}
+
+// plis_mlp_forward without any allocation: the feature rows and outputs are
+// staged through this device's scratch (at most kNoAllocRows rows).  Every
+// element the kernel reads is written by the same call first, so nothing
+// needs clearing; calls that overlap on one GPU race on the scratch.
+extern "C" int plis_mlp_forward_noalloc(plis_ctx* c, const float* feats, uint32_t n, float* out) {
+    if (!c || !feats || !out || n == 0) return PLIS_ERR_ARG;
+    if (n > kNoAllocRows) {
+        c->err = "plis_mlp_forward_noalloc: at most " + std::to_string(kNoAllocRows) + " rows per call, got " +
+                 std::to_string(n);
+        return PLIS_ERR_ARG;
+    }
+    if (!c->d_w32) {
+        c->err = "no network set";
+        return PLIS_ERR_STATE;
+    }
+    std::string serr;
+    float* base = scratch_ptr(c->device, serr);
+    if (!base) {
+        c->err = serr;
+        return PLIS_ERR_HIP;
+    }
+    float* d_f = base;
+    float* d_o = base + (size_t)kNoAllocRows * 12;
+    HIPCHK(c, hipMemcpyAsync(d_f, feats, (size_t)n * 12 * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    const MlpDev m{c->d_w32, c->d_q, c->d_norm};
+    const uint32_t tiles = (n + 63) / 64;
+    if (c->mlp_act == 1)
+        hipLaunchKernelGGL(mlp_infer_kernel<1>, dim3(tiles), dim3(64), 0, c->stream, m, d_f, n, d_o);
+    else
+        hipLaunchKernelGGL(mlp_infer_kernel<0>, dim3(tiles), dim3(64), 0, c->stream, m, d_f, n, d_o);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(out, d_o, (size_t)n * 2 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return PLIS_OK;
 }
 
