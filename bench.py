@@ -137,19 +137,23 @@ def gpu_measure(wl, steps, warmup, seed=2024, autotune=True, shape=None):
 
 
 def learned_measure(wl, steps, warmup, train_steps, batch, seed=2024, kappa=1.0, snapshots=16,
-                    per_snapshot=16384, shape=None):
+                    per_snapshot=16384, shape=None, act=1, records=None):
     """Records from plain steps of the same workload, GPU training, then the
-    learned step timed like the plain one."""
+    learned step timed like the plain one.  act: the network's activation
+    (1 GELU, 0 ReLU); records: (F, Y, L, plain_stats) to reuse."""
     import torch
 
     from pliscuda_amd import capi, learned
-    N = int(sum(t["N"] for t in wl["topos"]))
     rec = capi.Context(wl["net"], wl["topos"], wl["x0"], eps=wl["eps"], nc=wl["nc"], seed=seed + 1)
     t0 = time.perf_counter()
-    F, Y, Cc = learned.collect_records(rec, snapshots=snapshots, stride=4, per_snapshot=per_snapshot)
+    if records is None:
+        F, Y, Cc = learned.collect_records(rec, snapshots=snapshots, stride=4, per_snapshot=per_snapshot)
+        plain_stats = [rec.stats(r) for r in range(rec.n)]
+    else:
+        F, Y, Cc, plain_stats = records
     t_rec = time.perf_counter() - t0
-    plain_stats = [rec.stats(r) for r in range(rec.n)]
     vmax = learned.calibrate_viol_rate(plain_stats, [int(t["N"]) for t in wl["topos"]])
+    rec.mlp_activation(act)
     t0 = time.perf_counter()
     norm, losses = learned.train(rec, F, Y, Cc, steps=train_steps, batch=batch, lr=1e-3)
     rec.sync()
@@ -160,6 +164,7 @@ def learned_measure(wl, steps, warmup, train_steps, batch, seed=2024, kappa=1.0,
     ctx = capi.Context(wl["net"], wl["topos"], wl["x0"], eps=wl["eps"], nc=wl["nc"], seed=seed,
                        waves_per_block=wpb, blocks_per_cu=bpc)
     ctx.mlp_set(w, norm)
+    ctx.mlp_activation(act)
     ctx.set_tau_mode(True, kappa=kappa, theta=learned.THETA, viol_rate_max=vmax)
     ctx.arm(t_end=float("inf"))
     ctx.iterate(warmup)
@@ -172,7 +177,8 @@ def learned_measure(wl, steps, warmup, train_steps, batch, seed=2024, kappa=1.0,
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     s1 = [ctx.stats(r) for r in range(ctx.n)]
-    prof = ctx.profile(min(steps, 20))
+    n_prof = min(steps, 20)
+    prof = ctx.profile_kernels(n_prof)
     ctx.close()
     d = lambda f: sum(b[f] - a[f] for a, b in zip(s0, s1))
     props = d("learned_steps") + d("learned_rejects")
@@ -182,8 +188,12 @@ def learned_measure(wl, steps, warmup, train_steps, batch, seed=2024, kappa=1.0,
             "rejection_rate": d("learned_rejects") / props if props else None,
             "subvolume_simtime": sum((b["t"] - a["t"]) * n for a, b, n in
                                      zip(s0, s1, [int(t["N"]) for t in wl["topos"]])),
-            "tau_ms": prof[1] / min(steps, 20), "leap_ms": prof[3] / min(steps, 20),
-            "records": int(F.shape[0]), "records_seconds": t_rec, "train_steps": train_steps,
+            "activation": "gelu" if act == 1 else "relu",
+            "tau_ms": prof[1] / n_prof, "decide_ms": prof[2] / n_prof, "leap_ms": prof[3] / n_prof,
+            "commit_ms": prof[4] / n_prof, "leap_kernel_ms": prof[5] / n_prof, "redo_ms": prof[6] / n_prof,
+            "finish_with_proposals_ms": prof[7] / n_prof, "iteration_ms_events": prof[0] / n_prof,
+            "records": int(F.shape[0]), "reject_label_rate": float(Cc.mean()), "records_seconds": t_rec,
+            "train_steps": train_steps, "_records": (F, Y, Cc, plain_stats),
             "train_batch": batch, "train_seconds": t_train,
             "loss_first": float(losses[:10].mean()), "loss_last": float(losses[-100:].mean()),
             "viol_rate_max": vmax, "kappa": kappa}
@@ -255,6 +265,45 @@ def abc_to_t_end(seed=2024):
             "gpu_state_bit_identical": all(np.array_equal(g, x) for g, x in zip(gpu_states, xs))}
 
 
+def reference_benchmarks(reps=2, seed=2024):
+    """The reference's benchmark loop (PLISCuda/benchmarks/run.sh: each case
+    run NUM times through the binary, steps/second and elapsed time parsed
+    from stdout) over its four benchmark inputs, with the driver's extra
+    keys: subvolume-leaps/second and firings."""
+    import re
+    import shutil
+    import subprocess
+    import tempfile
+    root = os.path.join(ROOT, "tests", "golden", "ref_benchmarks")
+    driver = os.path.join(ROOT, "pliscuda_amd", "bin", "pliscuda")
+    res = {}
+    for name in ("chemotaxis", "diff-5125", "diff-64", "react-64"):
+        runs = []
+        with tempfile.TemporaryDirectory() as tmp:
+            for f in os.listdir(os.path.join(root, name)):
+                shutil.copyfile(os.path.join(root, name, f), os.path.join(tmp, f))
+            with open(os.path.join(tmp, "conf.txt"), "a") as f:
+                f.write(f"\nseed={seed}\n")
+            for _ in range(reps):
+                r = subprocess.run([driver, "conf.txt"], cwd=tmp, capture_output=True, text=True, timeout=120)
+                if r.returncode != 0:
+                    raise RuntimeError(f"driver failed on {name}: {r.stderr[-500:]}")
+                def num(key):
+                    m = re.search(re.escape(key) + r"\s*([0-9.eE+-]+)", r.stdout)
+                    return float(m.group(1)) if m else None
+                el = re.search(r"elapsed time:\s*(\d+)h\s*(\d+)m\s*([0-9.]+)s", r.stdout)
+                secs = int(el.group(1)) * 3600 + int(el.group(2)) * 60 + float(el.group(3)) if el else None
+                runs.append({"steps_per_s": num("steps/second:"), "elapsed_s": secs,
+                             "subvolume_leaps_per_s": num("subvolume-leaps/second:"),
+                             "firings": num("firings:"), "steps": num("simulation steps:")})
+        mean = lambda k: float(np.mean([q[k] for q in runs])) if all(q[k] is not None for q in runs) else None
+        res[name] = {"runs": reps, "steps_per_s": mean("steps_per_s"), "elapsed_s": mean("elapsed_s"),
+                     "subvolume_leaps_per_s": mean("subvolume_leaps_per_s"),
+                     "firings_per_s": (mean("firings") / mean("elapsed_s")) if mean("elapsed_s") else None,
+                     "steps": mean("steps")}
+    return res
+
+
 def copy_probe(nbytes=1 << 28, reps=5):
     """A plain device-to-device copy of known size (torch elementwise copy),
     used to calibrate the FETCH_SIZE / WRITE_SIZE counters."""
@@ -307,6 +356,8 @@ def main():
     ap.add_argument("--extra", action="store_true", help="also measure the Schlogl grid and batch")
     ap.add_argument("--copy-probe", action="store_true", help="only run the counter-calibration copy")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-ref-benchmarks", action="store_true",
+                    help="skip the loop over the reference's benchmark inputs")
     ap.add_argument("--train-steps", type=int, default=20000)
     ap.add_argument("--train-batch", type=int, default=65536)
     ap.add_argument("--kappa", type=float, default=1.0)
@@ -380,20 +431,33 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic},
     }
+    # the bound that applies: VALU issue when the kernel fills a larger
+    # fraction of the chip's VALU issue slots than of HBM bandwidth (the
+    # GB/s figures stay those of its algorithmic bytes)
+    vi = [v for v in out["valu_issue"] if v["kernel"] == kname]
+    if vi and vi[0]["frac"] > out["roofline"]["frac"]:
+        out["roofline"]["bound"] = "valu"
+        out["roofline"]["valu_issue_frac"] = vi[0]["frac"]
     if not args.no_learned and args.workload == "brusselator":
-        lr_ = learned_measure(wl, args.steps, args.warmup, args.train_steps, args.train_batch,
-                              kappa=args.kappa, shape=res["tuned"])
-        lv = lr_["subvolume_leaps"] / lr_["seconds"]
         simt_plain = res["subvolume_simtime"] / res["seconds"]
-        simt_learned = lr_["subvolume_simtime"] / lr_["seconds"]
-        out["learned"] = {"subvolume_leaps_per_s": lv, "firings_per_s": lr_["firings"] / lr_["seconds"],
-                          "ms_per_step": lr_["seconds"] * 1e3 / args.steps,
-                          "speedup_per_step": lv / value,
-                          "subvolume_simtime_per_s": simt_learned,
-                          "speedup_simulated_time": simt_learned / simt_plain if simt_plain else None,
-                          **{k: v for k, v in lr_.items() if k not in ("seconds",)}}
+        recs = None
+        # the network as specified (GELU), then the same records with ReLU
+        for key, act in (("learned", 1), ("learned_relu", 0)):
+            lr_ = learned_measure(wl, args.steps, args.warmup, args.train_steps, args.train_batch,
+                                  kappa=args.kappa, shape=res["tuned"], act=act, records=recs)
+            recs = lr_.pop("_records")
+            lv = lr_["subvolume_leaps"] / lr_["seconds"]
+            simt_learned = lr_["subvolume_simtime"] / lr_["seconds"]
+            out[key] = {"subvolume_leaps_per_s": lv, "firings_per_s": lr_["firings"] / lr_["seconds"],
+                        "ms_per_step": lr_["seconds"] * 1e3 / args.steps,
+                        "speedup_per_step": lv / value,
+                        "subvolume_simtime_per_s": simt_learned,
+                        "speedup_simulated_time": simt_learned / simt_plain if simt_plain else None,
+                        **{k: v for k, v in lr_.items() if k not in ("seconds",)}}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl, 2024, args.cpu_sample_steps)
+    if not args.no_ref_benchmarks:
+        out["reference_benchmarks"] = reference_benchmarks()
     if args.extra:
         extra = {}
         for name in ("schlogl", "batch"):
@@ -408,6 +472,7 @@ def main():
         wb = build_workload("bruss_batch")
         rp = gpu_measure(wb, 100, 10)
         rl = learned_measure(wb, 100, 10, 5000, 65536, snapshots=8, per_snapshot=512, shape=rp["tuned"])
+        rl.pop("_records")
         vp, vl = rp["subvolume_leaps"] / rp["seconds"], rl["subvolume_leaps"] / rl["seconds"]
         sp, sl = rp["subvolume_simtime"] / rp["seconds"], rl["subvolume_simtime"] / rl["seconds"]
         extra[wb["name"] + "_learned"] = {

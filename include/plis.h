@@ -167,6 +167,15 @@ int plis_restore(plis_ctx *ctx, const void *buf, uint64_t bytes);
  * not depend on it. */
 int plis_redo_tiles(plis_ctx *ctx, uint64_t *total);
 
+/* Diagnostic standing in for the reference's cuda-memcheck initcheck job
+ * (PLISCuda/test/run.sh:100-120): fill every device buffer that a step
+ * writes before it reads -- each realization's non-current count buffer,
+ * the tile partials of the tau and leap passes, the redo list -- with
+ * sentinels that are harmless if read but change the trajectory, and forget
+ * any learned proposals.  A run after it must equal a run without it bit for
+ * bit; a kernel that read an unwritten word would break that. */
+int plis_debug_poison(plis_ctx *ctx);
+
 /* Autotune waves-per-block x blocks-per-CU on this context's workload by
  * timing `iters` real iterations per candidate (results do not depend on
  * the launch shape, so the trajectory is unchanged). */

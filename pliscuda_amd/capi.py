@@ -24,7 +24,7 @@ EXPORTS = ("plis_create", "plis_run", "plis_run_until", "plis_arm", "plis_iterat
            "plis_launch_shape", "plis_redo_tiles", "plis_checkpoint_size", "plis_checkpoint", "plis_restore", "plis_autotune", "plis_destroy", "plis_last_error",
            "plis_device_count", "plis_mlp_set", "plis_mlp_get", "plis_mlp_forward",
            "plis_mlp_loss_grad", "plis_mlp_train", "plis_records", "plis_set_tau_mode",
-           "plis_mlp_activation", "plis_mlp_forward_noalloc")
+           "plis_mlp_activation", "plis_mlp_forward_noalloc", "plis_debug_poison")
 MLP_PARAMS = 5378
 
 
@@ -102,6 +102,7 @@ def _declare(L):
     L.plis_mlp_get.argtypes = [vp, fp]
     L.plis_mlp_forward.argtypes = [vp, fp, C.c_uint32, fp]
     L.plis_mlp_forward_noalloc.argtypes = [vp, fp, C.c_uint32, fp]
+    L.plis_debug_poison.argtypes = [vp]
     L.plis_mlp_loss_grad.argtypes = [vp, fp, fp, fp, C.c_uint32, fp, P(C.c_double)]
     L.plis_mlp_train.argtypes = [vp, fp, fp, fp, C.c_uint64, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64, fp]
     L.plis_records.argtypes = [vp, C.c_int32, fp, fp, fp]
@@ -227,6 +228,10 @@ class Context:
         """Continue from a checkpoint of a context with the same model and seed."""
         buf = C.create_string_buffer(bytes(blob), len(blob))
         self._chk(lib().plis_restore(self.h, buf, len(blob)), "plis_restore")
+
+    def debug_poison(self):
+        """Sentinels in every buffer a step writes before it reads (initcheck stand-in)."""
+        self._chk(lib().plis_debug_poison(self.h), "plis_debug_poison")
 
     def redo_tiles(self):
         """Tiles the edge-parallel leap passes have left to their redo pass

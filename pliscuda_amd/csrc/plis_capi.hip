@@ -103,6 +103,8 @@ struct plis_ctx {
     double theta = 0.0, viol_rate_max = 1.0, kappa = 1.0;
     int learned_on = 0;
     int mlp_act = 1;  // 0 ReLU, 1 GELU
+    // blocks per CU of the learned tau pass and of the learned leap-pass end
+    uint32_t learned_bpc = 1, finish_bpc = 4;
     float *d_w32 = nullptr, *d_m = nullptr, *d_v = nullptr, *d_norm = nullptr, *d_grad = nullptr;
     __bf16* d_q = nullptr;
     float* d_partial = nullptr;
@@ -359,12 +361,22 @@ static void launch_variant(plis_ctx* c, int which, const KernelArgs& a) {
     const dim3 grid(c->blocks), block(64 * c->wpb), rgrid(c->nreal), rblock(1024), dblock(decide_threads<MS>());
     const bool mc = c->nh.C > 1;  // several compartments: per-subvolume rate tables
     const bool gelu = c->mlp_act == 1;
+    // learned kernels stage the network in LDS once per block: fewer, longer-lived blocks
+    // the redo list is usually empty or short: two blocks per CU walk it
+    const dim3 rgrid2(std::max<uint32_t>(8u, std::min<uint32_t>(c->blocks, ((uint32_t)c->cus * 2u + 7u) & ~7u)));
+    const dim3 fgrid(std::max<uint32_t>(8u, std::min<uint32_t>(c->blocks, ((uint32_t)c->cus * c->finish_bpc + 7u) & ~7u)));
+    const dim3 lgrid(std::max<uint32_t>(8u, std::min<uint32_t>(c->blocks, ((uint32_t)c->cus * c->learned_bpc + 7u) & ~7u)));
     switch (which) {
         case 0:
             if constexpr (MS <= 8) {
                 if (c->learned_on) {
-                    if (gelu) hipLaunchKernelGGL((learned_tau_kernel<MS, MR, 1>), grid, block, 0, c->stream, a);
-                    else hipLaunchKernelGGL((learned_tau_kernel<MS, MR, 0>), grid, block, 0, c->stream, a);
+                    // one pass covers fallbacks and refreshes; in steady state
+                    // it skips every tile, so it runs on a small grid
+                    if (mc && gelu) hipLaunchKernelGGL((learned_tau_kernel<MS, MR, true, 1>), lgrid, block, 0, c->stream, a);
+                    else if (mc) hipLaunchKernelGGL((learned_tau_kernel<MS, MR, true, 0>), lgrid, block, 0, c->stream, a);
+                    else if (gelu) hipLaunchKernelGGL((learned_tau_kernel<MS, MR, false, 1>), lgrid, block, 0, c->stream, a);
+                    else hipLaunchKernelGGL((learned_tau_kernel<MS, MR, false, 0>), lgrid, block, 0, c->stream, a);
+                    break;
                 }
             }
             if (mc) hipLaunchKernelGGL((phase_a_kernel<MS, MR, true>), grid, block, 0, c->stream, a);
@@ -379,25 +391,25 @@ static void launch_variant(plis_ctx* c, int which, const KernelArgs& a) {
                     if (mc) {
                         hipLaunchKernelGGL((phase_b_kernel<MS, MR, true, true>), grid, block, 0, c->stream, a);
                         if (c->split_ev) (void)hipEventRecord(c->split_ev[0], c->stream);
-                        hipLaunchKernelGGL((phase_b_redo_kernel<MS, MR, true>), grid, block, 0, c->stream, a);
+                        hipLaunchKernelGGL((phase_b_redo_kernel<MS, MR, true>), rgrid2, block, 0, c->stream, a);
                         if (c->split_ev) (void)hipEventRecord(c->split_ev[1], c->stream);
                         if (!c->learned_on)
                             hipLaunchKernelGGL((phase_b_finish_kernel<MS, MR, true>), grid, block, 0, c->stream, a);
                         else if (gelu)
-                            hipLaunchKernelGGL((learned_finish_kernel<MS, MR, true, 1>), grid, block, 0, c->stream, a);
+                            hipLaunchKernelGGL((learned_finish_kernel<MS, MR, true, 1>), fgrid, block, 0, c->stream, a);
                         else
-                            hipLaunchKernelGGL((learned_finish_kernel<MS, MR, true, 0>), grid, block, 0, c->stream, a);
+                            hipLaunchKernelGGL((learned_finish_kernel<MS, MR, true, 0>), fgrid, block, 0, c->stream, a);
                     } else {
                         hipLaunchKernelGGL((phase_b_kernel<MS, MR, true, false>), grid, block, 0, c->stream, a);
                         if (c->split_ev) (void)hipEventRecord(c->split_ev[0], c->stream);
-                        hipLaunchKernelGGL((phase_b_redo_kernel<MS, MR, false>), grid, block, 0, c->stream, a);
+                        hipLaunchKernelGGL((phase_b_redo_kernel<MS, MR, false>), rgrid2, block, 0, c->stream, a);
                         if (c->split_ev) (void)hipEventRecord(c->split_ev[1], c->stream);
                         if (!c->learned_on)
                             hipLaunchKernelGGL((phase_b_finish_kernel<MS, MR, false>), grid, block, 0, c->stream, a);
                         else if (gelu)
-                            hipLaunchKernelGGL((learned_finish_kernel<MS, MR, false, 1>), grid, block, 0, c->stream, a);
+                            hipLaunchKernelGGL((learned_finish_kernel<MS, MR, false, 1>), fgrid, block, 0, c->stream, a);
                         else
-                            hipLaunchKernelGGL((learned_finish_kernel<MS, MR, false, 0>), grid, block, 0, c->stream, a);
+                            hipLaunchKernelGGL((learned_finish_kernel<MS, MR, false, 0>), fgrid, block, 0, c->stream, a);
                     }
                     break;
                 }
@@ -1062,6 +1074,36 @@ extern "C" int plis_launch_shape(plis_ctx* c, int32_t* out) {
     out[1] = (int32_t)c->blocks;
     out[2] = (int32_t)c->ntiles;
     return PLIS_OK;
+}
+
+// Sentinels of plis_debug_poison: harmless if read (no index leaves its
+// range), but a read changes the trajectory -- a tiny tau in the tau
+// partials, firings and a negative flag in the leap partials, tile 0 in the
+// redo list (its open draws would be added twice), huge counts in the
+// non-current count buffers.
+constexpr uint32_t kPoisonCount = 0x7FFFFFF0u;
+
+__global__ __launch_bounds__(256) void poison_parts_kernel(PartA* pa, PartB* pb, uint32_t* redo, uint32_t ntiles) {
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < ntiles; t += gridDim.x * blockDim.x) {
+        pa[t] = PartA{1e-300, 1e-300, 0u, 0xDEADu};
+        pb[t] = PartB{1ull << 40, 1u, 7u};
+        redo[t] = 0u;
+    }
+}
+
+extern "C" int plis_debug_poison(plis_ctx* c) {
+    if (!c) return PLIS_ERR_ARG;
+    if (int rc = pull_ctl(c)) return rc;
+    for (uint32_t r = 0; r < c->nreal; ++r) {
+        const RealInfo& ri = c->info[r];
+        uint32_t* other = c->ctl_host[r].cur ? c->d_x0 : c->d_x1;
+        HIPCHK(c, hipMemsetD32Async(other + ri.x_off, (int)kPoisonCount, (size_t)c->S * ri.N, c->stream));
+    }
+    hipLaunchKernelGGL(poison_parts_kernel, dim3(std::min<uint32_t>((c->ntiles + 255) / 256, 1024u)), dim3(256), 0,
+                       c->stream, c->d_pa_t, c->d_pb_t, c->d_redo, c->ntiles);
+    HIPCHK(c, hipGetLastError());
+    // the partials no longer hold any proposal
+    return invalidate_props(c);
 }
 
 extern "C" int plis_redo_tiles(plis_ctx* c, uint64_t* total) {

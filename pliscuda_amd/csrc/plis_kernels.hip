@@ -197,11 +197,14 @@ __device__ __forceinline__ bool is_crit(const uint32_t (&crit)[(MR + 31) / 32], 
 // products.  A negative or overflowing count breaks it too.  av / crit:
 // propensities and critical flags of x.
 constexpr double kLcSlack = 1.0 + 0x1p-30;
+// av_new: a_j(x') when the caller has it (same operations, same values),
+// else nullptr and the products are formed here.
 template <int MS, int MR>
 __device__ __forceinline__ uint32_t leap_condition_broken(const NetView& n, int c, int nc, double eps, double theta,
                                                           const uint32_t (&x)[MS], const long long (&nx)[MS],
                                                           const double (&av)[MR],
-                                                          const uint32_t (&crit)[(MR + 31) / 32]) {
+                                                          const uint32_t (&crit)[(MR + 31) / 32],
+                                                          const double* av_new = nullptr) {
     uint32_t xn[MS];
     bool neg = false;
 #pragma unroll
@@ -216,14 +219,19 @@ __device__ __forceinline__ uint32_t leap_condition_broken(const NetView& n, int 
         if (j >= n.R) break;
         const int nr = n.nrs[j];
         if (is_crit<MR>(crit, j) || !(av[j] > 0.0) || nr == 0) continue;
-        double aj = n.k[c * n.R + j];
-        for (int i = 0; i < nr; ++i) {
-            const int sp = n.rs[j * 3 + i], r = n.rr[j * 3 + i];
-            uint32_t xs = 0;
+        double aj;
+        if (av_new) {
+            aj = av_new[j];
+        } else {
+            aj = n.k[c * n.R + j];
+            for (int i = 0; i < nr; ++i) {
+                const int sp = n.rs[j * 3 + i], r = n.rr[j * 3 + i];
+                uint32_t xs = 0;
 #pragma unroll
-            for (int q = 0; q < MS; ++q)
-                if (q == sp) xs = xn[q];
-            aj = aj * binom_d(xs, r);
+                for (int q = 0; q < MS; ++q)
+                    if (q == sp) xs = xn[q];
+                aj = aj * binom_d(xs, r);
+            }
         }
         const double D = __builtin_fabs(aj - av[j]);
         const double t = (theta * av[j]) * kLcSlack;
@@ -407,15 +415,19 @@ __device__ __forceinline__ void tau_own(const KernelArgs& a, const NetView& n, c
 }
 
 // In-hops (nu = +1) from one in-neighbour, species whose hops are non-critical:
-// du / xu are the neighbour's diffusion constants and counts.
+// du / xu are the neighbour's diffusion constants and counts.  A critical or
+// absent hop (count below max(n_c, 1), or d = 0) adds +0 instead of being
+// skipped: the sums of the active terms are the same values (only a -0 sum
+// would turn into +0, and tau' uses |mu|), without a branch per term.
 template <int MS>
 __device__ __forceinline__ void tau_in_edge(int S, const double (&du)[MS], const uint32_t (&xu)[MS], double wq,
                                             int nc, TauAcc<MS>& t) {
+    const uint32_t nc1 = nc > 1 ? (uint32_t)nc : 1u;
 #pragma unroll
     for (int s = 0; s < MS; ++s) {
         if (s >= S) break;
-        if (!hop_active(du[s], xu[s], nc)) continue;
-        const double ain = (du[s] * (double)xu[s]) * wq;
+        const uint32_t xe = xu[s] >= nc1 ? xu[s] : 0u;
+        const double ain = (du[s] * (double)xe) * wq;
         t.mu[s] += ain;
         t.sig[s] += ain;
     }
@@ -535,10 +547,39 @@ __device__ void finalize_A(const KernelArgs& a, const NetView& n, const RealInfo
     if (lead) a.ctl[r] = c;
 }
 
+// Cao's tau' / tau'' of one tile's subvolumes into the tile partial (the
+// debug outputs, when given, get every subvolume's values instead).
+template <int MS, int MR>
+__device__ __forceinline__ void cao_tau_tile(const KernelArgs& a, const NetView& n, const RealInfo& ri,
+                                             const cptr<RealCtl> cp, const TileEnt& te, uint32_t t, bool dbg) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t cur = wave_uniform(cp->cur);
+    const uint32_t* X = cur ? a.x1 : a.x0;
+    const uint32_t v = te.v0 + lane;
+    double tp = kInf, t2 = kInf, a0c = 0.0;
+    uint32_t v2 = v;
+    if (v < ri.N) {
+        tau_subvolume<MS, MR>(a, n, ri, X, v, tp, a0c);
+        if (a0c > 0.0) {
+            const uint32_t step = wave_uniform(cp->step);
+            const U4 o = philox10(v, kSelfId, step, ctr_word3(ri.id, 0u, kTau2, 0u), a.seed_lo, a.seed_hi);
+            t2 = -dlog(uniform01(o.x)) / a0c;
+        }
+        if (dbg) {
+            const uint64_t base = ri.x_off / (uint64_t)n.S;
+            a.dbg_tp[base + v] = tp;
+            a.dbg_a0c[base + v] = a0c;
+            a.dbg_t2[base + v] = t2;
+        }
+    }
+    if (dbg) return;
+    reduce_partA(tp, t2, v2);
+    if (lane == 0) a.partA_tile[t] = PartA{tp, t2, v2, 0u};
+}
+
 template <int MS, int MR, bool MC>
 __global__ __launch_bounds__(256) void phase_a_kernel(KernelArgs a) {
     const NetView n = net_view<MC>(a);
-    const uint32_t lane = threadIdx.x & 63u;
     for (uint32_t it = 0;; ++it) {
         const uint32_t t = tile_at(it, a.ntiles, a.waves_per_block);
         if (t == 0xFFFFFFFFu) break;
@@ -549,30 +590,7 @@ __global__ __launch_bounds__(256) void phase_a_kernel(KernelArgs a) {
         const uint32_t done = wave_uniform(cp->done), retry = wave_uniform(cp->retry);
         const bool dbg = a.dbg_tp != nullptr;
         if (!dbg && (done || retry)) continue;
-        if (!dbg && a.learned_on && !wave_uniform(cp->fallback)) continue;  // learned pass
-        const uint32_t cur = wave_uniform(cp->cur);
-        const uint32_t* X = cur ? a.x1 : a.x0;
-        const uint32_t v = te.v0 + lane;
-        double tp = kInf, t2 = kInf, a0c = 0.0;
-        uint32_t v2 = v;
-        if (v < ri.N) {
-            tau_subvolume<MS, MR>(a, n, ri, X, v, tp, a0c);
-            if (a0c > 0.0) {
-                const uint32_t step = wave_uniform(cp->step);
-                const U4 o = philox10(v, kSelfId, step, ctr_word3(ri.id, 0u, kTau2, 0u), a.seed_lo,
-                                      a.seed_hi);
-                t2 = -dlog(uniform01(o.x)) / a0c;
-            }
-            if (dbg) {
-                const uint64_t base = ri.x_off / (uint64_t)n.S;
-                a.dbg_tp[base + v] = tp;
-                a.dbg_a0c[base + v] = a0c;
-                a.dbg_t2[base + v] = t2;
-            }
-        }
-        if (dbg) continue;
-        reduce_partA(tp, t2, v2);
-        if (lane == 0) a.partA_tile[t] = PartA{tp, t2, v2, 0u};
+        cao_tau_tile<MS, MR>(a, n, ri, cp, te, t, dbg);
     }
 }
 
@@ -834,34 +852,37 @@ __device__ __forceinline__ bool leap_subvolume(const KernelArgs& a, const NetVie
 // atomics keep every sum exact and order-free.  phase_b_finish_kernel adds
 // the balances to the counts once every draw of the pass is in.
 //
-// Two tiers of draws.  phase_b_kernel's walk runs only the fast samplers
-// (zero shortcut and float squeeze) and applies what they settle; a draw
-// they leave open goes to the wavefront's LDS queue as a 32-bit code and is
-// drawn after the walk with the double search, inline, where almost nothing
-// else is live.  A tile whose queue overflows, or that holds a draw of mean
-// >= 10 (PTRS), applies its settled draws only and goes on the redo list:
-// phase_b_redo_kernel walks it again and draws exactly the draws the fast
-// samplers leave open (the same test on the same inputs) with the complete
-// sampler.  Draws are functions of their counters, so the two passes give
-// the tile the values one pass would.  No call and no PTRS in phase_b_kernel
-// keeps its register footprint, and so its occupancy, at that of the fast
-// path.
+// Compacted draws.  Almost every draw of a step is 0 (hop means are ~1e-3
+// on the headline mesh), so phase_b_kernel's walk only applies the zero
+// shortcut: a draw it cannot settle as 0 goes to the wavefront's LDS queue
+// as its code and Philox word, and after the walk the whole wavefront draws
+// the queued ones, one per lane -- the float squeeze, then the double
+// search where the squeeze cannot decide -- so the search runs once per 64
+// non-zero draws instead of once per draw site.  A tile whose queue
+// overflows, or that holds a draw of mean >= 10 (PTRS), applies nothing
+// from its queue and goes on the redo list: phase_b_redo_kernel walks it
+// again and draws every draw the zero shortcut leaves open (the same test
+// on the same inputs) with the complete sampler.  Draws are functions of
+// their counters, so the two passes give the tile the values one pass
+// would.  No call and no PTRS in phase_b_kernel keeps its register
+// footprint, and so its occupancy, at that of the walk.
 // ---------------------------------------------------------------------------
-constexpr uint32_t kQueue = 128;               // codes per wavefront
+constexpr uint32_t kQueue = 256;               // queued draws per wavefront
 constexpr uint32_t kQueueReact = 0x80000000u;  // reaction code: flag | lane << 8 | j
 constexpr uint32_t kQueueDefer = kQueue + 1u;  // count bump of a draw the queue does not take
 constexpr float kPtrsSureF = 10.0001f;         // a float mean above it is a double mean >= 10
 constexpr uint32_t kMaxTileEdges = 1u << 27;   // hop codes hold the entry offset in 27 bits
 
 // A mean certainly >= 10 (PTRS) sends the tile to the redo pass; others
-// are queued (the drain re-forms the mean in double and may still defer).
-__device__ __forceinline__ void q_append(uint32_t* s_q, uint32_t* s_qn, bool ptrs, uint32_t code) {
+// are queued with their word (the drain re-forms the mean and may still
+// defer).
+__device__ __forceinline__ void q_append(uint2* s_q, uint32_t* s_qn, bool ptrs, uint32_t code, uint32_t word) {
     if (ptrs) {
         atomicAdd(s_qn, kQueueDefer);
         return;
     }
     const uint32_t slot = atomicAdd(s_qn, 1u);
-    if (slot < kQueue) s_q[slot] = code;
+    if (slot < kQueue) s_q[slot] = make_uint2(code, word);
 }
 
 // Owner of entry e of the tile: last i with s_rp[i] <= e.
@@ -895,14 +916,14 @@ __device__ __forceinline__ void add_balance(const KernelArgs& a, const RealInfo&
 }
 
 // The lane's draws: its forward entries (both directions of every species
-// of an undirected edge), then the reactions of its subvolume.  Fast walk:
-// settled draws applied, open ones queued.  REDO: the open draws only, with
-// the complete sampler.
+// of an undirected edge), then the reactions of its subvolume.  Walk: the
+// zero shortcut, draws it leaves open queued.  REDO: the open draws, with
+// the complete sampler, applied.
 template <int MS, int MR, bool REDO>
 __device__ __forceinline__ void walk_fwd_draws(const KernelArgs& a, const NetView& n, const RealInfo& ri,
                                                const uint32_t* X, uint32_t v0, const StepView& sv,
                                                const uint32_t (*s_x)[kTile], unsigned long long (*s_del)[kTile],
-                                               const uint32_t* s_rp, const int32_t* s_c, uint32_t* s_q,
+                                               const uint32_t* s_rp, const int32_t* s_c, uint2* s_q,
                                                uint32_t* s_qn, unsigned long long& fire) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t N = ri.N;
@@ -915,9 +936,6 @@ __device__ __forceinline__ void walk_fwd_draws(const KernelArgs& a, const NetVie
     const uint32_t k0 = a.seed_lo, k1 = a.seed_hi;
     const uint32_t c3hop = ctr_word3(sv.real, sv.attempt, kHop, 0u);
     const uint32_t eb = s_rp[0], ee = s_rp[kTile];
-    // fast-path firings of this lane: each settled draw is <= 15 and a tile
-    // has < 2^27 entries, so 32 bits hold them
-    uint32_t fire_fast = 0u;
     // software pipeline: an entry's neighbour counts are requested one
     // iteration ahead, its neighbour id and weight two iterations ahead
     // (counts of a neighbour inside the tile are read from global memory
@@ -971,43 +989,36 @@ __device__ __forceinline__ void walk_fwd_draws(const KernelArgs& a, const NetVie
             const float lr1 = two ? hop_mean_f(du1, xu1, wf, tauf, nc) : 0.0f;
             if (!(lf0 > 0.0f || lf1 > 0.0f || lr0 > 0.0f || lr1 > 0.0f)) continue;
             const U4 o = philox10(vo, u, sv.step, cb, k0, k1);
-            std::conditional_t<REDO, long long, int> f0 = 0, r0 = 0, f1 = 0, r1 = 0;  // forward / reverse hops
             if constexpr (REDO) {
+                long long f0 = 0, r0 = 0, f1 = 0, r1 = 0;  // forward / reverse hops
                 const double wq = (double)wf;
-                if (poisson_fast_f(lf0, o.x) < 0)
+                if (poisson_zero_f(lf0, o.x) < 0)
                     f0 = poisson(hop_mean(do0, xo0, wq, tau, nc), o.x, DrawId{vo, u, sv.step, cb, k0, k1, 0u});
-                if (poisson_fast_f(lr0, o.y) < 0)
+                if (poisson_zero_f(lr0, o.y) < 0)
                     r0 = poisson(hop_mean(du0, xu0, wq, tau, nc), o.y, DrawId{vo, u, sv.step, cb, k0, k1, 1u});
-                if (poisson_fast_f(lf1, o.z) < 0)
+                if (poisson_zero_f(lf1, o.z) < 0)
                     f1 = poisson(hop_mean(do1, xo1, wq, tau, nc), o.z, DrawId{vo, u, sv.step, cb, k0, k1, 2u});
-                if (poisson_fast_f(lr1, o.w) < 0)
+                if (poisson_zero_f(lr1, o.w) < 0)
                     r1 = poisson(hop_mean(du1, xu1, wq, tau, nc), o.w, DrawId{vo, u, sv.step, cb, k0, k1, 3u});
                 fire += (unsigned long long)(f0 + r0 + f1 + r1);
+                const long long b0 = r0 - f0, b1 = r1 - f1;
+                if (b0 != 0) {
+                    atomicAdd(&s_del[s0][i], (unsigned long long)b0);
+                    add_balance(a, ri, s_del, v0, s0, u, -b0);
+                }
+                if (b1 != 0) {
+                    atomicAdd(&s_del[s1][i], (unsigned long long)b1);
+                    add_balance(a, ri, s_del, v0, s1, u, -b1);
+                }
             } else {
                 const uint32_t wcode = ecode | (blk << 2);
-                int k;
-                if ((k = poisson_fast_f(lf0, o.x)) < 0) q_append(s_q, s_qn, lf0 > kPtrsSureF, wcode | 0u);
-                else f0 = k;
-                if ((k = poisson_fast_f(lr0, o.y)) < 0) q_append(s_q, s_qn, lr0 > kPtrsSureF, wcode | 1u);
-                else r0 = k;
-                if ((k = poisson_fast_f(lf1, o.z)) < 0) q_append(s_q, s_qn, lf1 > kPtrsSureF, wcode | 2u);
-                else f1 = k;
-                if ((k = poisson_fast_f(lr1, o.w)) < 0) q_append(s_q, s_qn, lr1 > kPtrsSureF, wcode | 3u);
-                else r1 = k;
-                fire_fast += (uint32_t)(f0 + r0 + f1 + r1);
-            }
-            const long long b0 = (long long)r0 - (long long)f0, b1 = (long long)r1 - (long long)f1;
-            if (b0 != 0) {
-                atomicAdd(&s_del[s0][i], (unsigned long long)b0);
-                add_balance(a, ri, s_del, v0, s0, u, -b0);
-            }
-            if (b1 != 0) {
-                atomicAdd(&s_del[s1][i], (unsigned long long)b1);
-                add_balance(a, ri, s_del, v0, s1, u, -b1);
+                if (poisson_zero_f(lf0, o.x) < 0) q_append(s_q, s_qn, lf0 > kPtrsSureF, wcode | 0u, o.x);
+                if (poisson_zero_f(lr0, o.y) < 0) q_append(s_q, s_qn, lr0 > kPtrsSureF, wcode | 1u, o.y);
+                if (poisson_zero_f(lf1, o.z) < 0) q_append(s_q, s_qn, lf1 > kPtrsSureF, wcode | 2u, o.z);
+                if (poisson_zero_f(lr1, o.w) < 0) q_append(s_q, s_qn, lr1 > kPtrsSureF, wcode | 3u, o.w);
             }
         }
     }
-    fire += fire_fast;
     const uint32_t v = v0 + lane;
     if (v >= N) return;
     // reactions of subvolume v, propensities and criticality as react_props
@@ -1035,14 +1046,13 @@ __device__ __forceinline__ void walk_fwd_draws(const KernelArgs& a, const NetVie
         }
         const uint32_t wd = (uint32_t)j & 3u;
         const double lam = tau * aj;
+        if (poisson_zero(lam, pick(o, wd)) >= 0) continue;
         long long k;
-        const int kf = poisson_fast(lam, pick(o, wd));
         if constexpr (REDO) {
-            if (kf >= 0) continue;
             k = poisson(lam, pick(o, wd), DrawId{v, kSelfId, sv.step, c3base | b, k0, k1, wd});
         } else {
-            if (kf < 0) q_append(s_q, s_qn, !(lam < kPoissonInvMax), kQueueReact | (lane << 8) | (uint32_t)j);
-            k = kf < 0 ? 0 : kf;
+            q_append(s_q, s_qn, !(lam < kPoissonInvMax), kQueueReact | (lane << 8) | (uint32_t)j, pick(o, wd));
+            continue;
         }
         if (k == 0) continue;
         fire += (unsigned long long)k;
@@ -1060,6 +1070,7 @@ struct QueuedHop {
     int s;
     bool fwd;  // the hop v0 + i -> u
     double lam;
+    float lf;  // the mean as the walk formed it in single precision
 };
 
 template <int MS>
@@ -1075,34 +1086,42 @@ __device__ __forceinline__ QueuedHop queued_hop(const KernelArgs& a, const NetVi
     h.u = a.fcol[ri.fe_off + e];
     h.s = 2 * (int)h.blk + (int)(h.wd >> 1);
     h.fwd = (h.wd & 1u) == 0u;
-    const double wq = (ri.flags & 2u) ? (double)a.fw[ri.fe_off + e] : 1.0;
+    const float wf = (ri.flags & 2u) ? a.fw[ri.fe_off + e] : 1.0f;
+    const double wq = (double)wf;
+    double d;
+    uint32_t xs;
     if (h.fwd) {
-        h.lam = hop_mean(n.d[(n.C > 1 ? s_c[h.i] : 0) * n.S + h.s], s_x[h.s][h.i], wq, sv.tau, a.n_critical);
+        d = n.d[(n.C > 1 ? s_c[h.i] : 0) * n.S + h.s];
+        xs = s_x[h.s][h.i];
     } else {
         const bool local = h.u - v0 < (uint32_t)kTile;
         const int cu = n.C > 1 ? (local ? s_c[h.u - v0] : a.comp[ri.comp_off + h.u]) : 0;
-        const uint32_t xu = local ? s_x[h.s][h.u - v0] : X[ri.x_off + (uint64_t)h.s * ri.N + h.u];
-        h.lam = hop_mean(n.d[cu * n.S + h.s], xu, wq, sv.tau, a.n_critical);
+        d = n.d[cu * n.S + h.s];
+        xs = local ? s_x[h.s][h.u - v0] : X[ri.x_off + (uint64_t)h.s * ri.N + h.u];
     }
+    h.lam = hop_mean(d, xs, wq, sv.tau, a.n_critical);
+    h.lf = hop_mean_f(d, xs, wf, (float)sv.tau, a.n_critical);
     return h;
 }
 
-// One queued draw with its mean in double: the double search (the drain
-// runs only when every queued mean is below 10).
+// One queued draw: the float squeeze on the mean and uniform the walk
+// formed, the double search where it cannot decide (the drain runs only
+// when every queued mean is below 10), applied to the balances.
 template <int MS>
 __device__ __forceinline__ void drain_fwd_draw(const KernelArgs& a, const NetView& n, const RealInfo& ri,
                                                const uint32_t* X, uint32_t v0, const StepView& sv,
                                                const uint32_t (*s_x)[kTile], unsigned long long (*s_del)[kTile],
-                                               const uint32_t* s_rp, const int32_t* s_c, uint32_t code,
+                                               const uint32_t* s_rp, const int32_t* s_c, uint2 q,
                                                unsigned long long& fire) {
-    const uint32_t k0 = a.seed_lo, k1 = a.seed_hi;
+    const uint32_t code = q.x, word = q.y;
     if (code & kQueueReact) {
         const uint32_t i = (code >> 8) & 63u, j = code & 255u;
         double aj = n.k[(n.C > 1 ? s_c[i] : 0) * n.R + (int)j];
         const int nr = n.nrs[j];
-        for (int q = 0; q < nr; ++q) aj = aj * binom_d(s_x[n.rs[j * 3 + q]][i], n.rr[j * 3 + q]);
-        const U4 o = philox10(v0 + i, kSelfId, sv.step, ctr_word3(sv.real, sv.attempt, kReact, 0u) | (j >> 2), k0, k1);
-        const long long k = inversion_search(sv.tau * aj, pick(o, j & 3u));
+        for (int r = 0; r < nr; ++r) aj = aj * binom_d(s_x[n.rs[j * 3 + r]][i], n.rr[j * 3 + r]);
+        const double lam = sv.tau * aj;
+        long long k = poisson_squeeze((float)lam, (float)uniform01(word));
+        if (k < 0) k = inversion_search(lam, word);
         if (k == 0) return;
         fire += (unsigned long long)k;
         for (int s = 0; s < n.S; ++s) {
@@ -1112,8 +1131,8 @@ __device__ __forceinline__ void drain_fwd_draw(const KernelArgs& a, const NetVie
         return;
     }
     const QueuedHop h = queued_hop<MS>(a, n, ri, X, v0, sv, s_x, s_rp, s_c, code);
-    const U4 o = philox10(v0 + h.i, h.u, sv.step, ctr_word3(sv.real, sv.attempt, kHop, 0u) | h.blk, k0, k1);
-    const long long k = inversion_search(h.lam, pick(o, h.wd));
+    long long k = h.lf < kSqueezeMaxF ? poisson_squeeze(h.lf, __uint2float_rn(word) * 0x1p-32f) : -1;
+    if (k < 0) k = inversion_search(h.lam, word);
     if (k == 0) return;
     fire += (unsigned long long)k;
     const long long b = h.fwd ? -k : k;  // balance of v0 + i
@@ -1128,7 +1147,7 @@ template <int MS, int MR, bool REDO>
 __device__ __forceinline__ bool leap_tile_fwd(const KernelArgs& a, const NetView& n, const RealInfo& ri,
                                               const uint32_t* X, uint32_t v0, const StepView& sv,
                                               uint32_t (*s_x)[kTile], unsigned long long (*s_del)[kTile],
-                                              uint32_t* s_rp, int32_t* s_c, uint32_t* s_q, uint32_t* s_qn,
+                                              uint32_t* s_rp, int32_t* s_c, uint2* s_q, uint32_t* s_qn,
                                               unsigned long long& fire) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t N = ri.N;
@@ -1157,7 +1176,7 @@ __device__ __forceinline__ bool leap_tile_fwd(const KernelArgs& a, const NetView
             // queued is drawn then, the redo pass draws all of it
             bool big = false;
             for (uint32_t q = lane; q < cnt; q += (uint32_t)kTile) {
-                const uint32_t code = s_q[q];
+                const uint32_t code = s_q[q].x;
                 if (!(code & kQueueReact))
                     big = big || !(queued_hop<MS>(a, n, ri, X, v0, sv, s_x, s_rp, s_c, code).lam < kPoissonInvMax);
             }
@@ -1185,7 +1204,7 @@ template <int MS, int MR>
 __device__ __forceinline__ void finish_tile(const KernelArgs& a, const NetView& n, const RealInfo& ri,
                                             const uint32_t* X, uint32_t* Xn, uint32_t v0, const StepView& sv,
                                             unsigned long long& fire, uint32_t& bad, uint32_t& viol,
-                                            uint32_t (&x)[MS], long long (&nx)[MS]) {
+                                            uint32_t (&x)[MS], long long (&nx)[MS], bool judge) {
     const uint32_t v = v0 + (threadIdx.x & 63u);
 #pragma unroll
     for (int s = 0; s < MS; ++s) {
@@ -1215,7 +1234,7 @@ __device__ __forceinline__ void finish_tile(const KernelArgs& a, const NetView& 
         nx[s] = y;
     }
     bad = neg ? 1u : 0u;
-    if (a.judge_on) {
+    if (judge) {
         const int c = n.C > 1 ? a.comp[ri.comp_off + v] : 0;
         double av[MR];
         uint32_t crit[(MR + 31) / 32];
@@ -1313,7 +1332,7 @@ __device__ __forceinline__ StepView step_view(const RealInfo& ri, cptr<RealCtl> 
 template <int MS, int MR, bool EP, bool REDO>
 __device__ __forceinline__ void phase_b_tile(const KernelArgs& a, const NetView& n, uint32_t t,
                                              uint32_t (*s_x)[kTile], unsigned long long (*s_del)[kTile],
-                                             uint32_t* s_rp, int32_t* s_c, uint32_t* s_q, uint32_t* s_qn) {
+                                             uint32_t* s_rp, int32_t* s_c, uint2* s_q, uint32_t* s_qn) {
     const uint32_t lane = threadIdx.x & 63u;
     const TileEnt te = ld_ro(a.tiles + t);
     const uint32_t r = wave_uniform(te.r);
@@ -1359,7 +1378,7 @@ void phase_b_kernel(KernelArgs a) {
     __shared__ unsigned long long s_del[4][LS][kTile];
     __shared__ uint32_t s_rp[4][kTile + 1];
     __shared__ int32_t s_c[4][kTile];
-    __shared__ uint32_t s_q[4][EP ? kQueue : 1];
+    __shared__ uint2 s_q[4][EP ? kQueue : 1];
     __shared__ uint32_t s_qn[4];
     for (uint32_t it = 0;; ++it) {
         const uint32_t t = tile_at(it, a.ntiles, a.waves_per_block);
@@ -1406,7 +1425,8 @@ __global__ __launch_bounds__(256) void phase_b_finish_kernel(KernelArgs a) {
         uint32_t bad = 0, viol = 0;
         uint32_t x[MS];
         long long nx[MS];
-        finish_tile<MS, MR>(a, n, ri, cur ? a.x1 : a.x0, cur ? a.x0 : a.x1, te.v0, sv, fire, bad, viol, x, nx);
+        finish_tile<MS, MR>(a, n, ri, cur ? a.x1 : a.x0, cur ? a.x0 : a.x1, te.v0, sv, fire, bad, viol, x, nx,
+                            a.judge_on != 0);
         reduce_partB(fire, bad, viol);
         if (lane == 0) a.partB_tile[t] = PartB{a.partB_tile[t].fire + fire, bad, viol};
     }

@@ -31,8 +31,16 @@ namespace plis {
 // Philox purpose of the records' trial leaps (the step's purposes are 1..5).
 constexpr uint32_t kTrial = 7u;
 
-// Features of one subvolume from its counts (the host restatement computes
-// the same quantities in double; single-precision hardware logs are enough
+// Whether species s (a compile-time index) is a reactant of reaction j:
+// scalar tests on the network tables.
+__device__ __forceinline__ bool is_reactant(const NetView& n, int j, int s) {
+    const int nr = n.nrs[j];
+    return (nr > 0 && n.rs[j * 3] == s) || (nr > 1 && n.rs[j * 3 + 1] == s) || (nr > 2 && n.rs[j * 3 + 2] == s);
+}
+
+// Features of one subvolume from its counts x and their propensities av /
+// critical flags crit (react_props; the host restatement computes the same
+// quantities in double, single-precision sums and hardware logs are enough
 // for a bf16 network):
 //   f[s] = log(1 + x_s), f[5+s] = log(1 + sum_{j: r_sj>0} a_j + d_s x_s W),
 //   f[10] = highest order among reactions with a_j > 0, f[11] = log prev tau.
@@ -42,14 +50,12 @@ constexpr uint32_t kTrial = 7u;
 // possible event stops as in the plain method.
 template <int MS, int MR>
 __device__ __forceinline__ void features_and_crit(const KernelArgs& a, const NetView& n, const RealInfo& ri,
-                                                  const uint32_t (&x)[MS], uint32_t v, float prev_logtau,
-                                                  float (&f)[12], double& a0c_out, bool& live) {
+                                                  const uint32_t (&x)[MS], const double (&av)[MR],
+                                                  const uint32_t (&crit)[(MR + 31) / 32], uint32_t v,
+                                                  float prev_logtau, float (&f)[12], double& a0c_out, bool& live) {
     const int c = n.C > 1 ? a.comp[ri.comp_off + v] : 0;
-    double av[MR];
-    uint32_t crit[(MR + 31) / 32];
-    react_props<MR, MS>(n, x, c, a.n_critical, av, crit);
     const double W = a.wsum[ri.rp_off + v];
-    double A[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    float A[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     int maxo = 0;
     double a0c = 0.0;
     bool lv = false;
@@ -59,14 +65,12 @@ __device__ __forceinline__ void features_and_crit(const KernelArgs& a, const Net
         if (av[j] > 0.0 && n.order[j] > maxo) maxo = n.order[j];
         if (is_crit<MR>(crit, j)) a0c += av[j];
         else if (av[j] > 0.0) lv = true;
-        const int nr = n.nrs[j];
-        for (int i = 0; i < nr; ++i) {
-            const int s = n.rs[j * 3 + i];
+        const float aj = (float)av[j];
 #pragma unroll
-            for (int q = 0; q < 5; ++q)
-                if (q == s) A[q] += av[j];
-        }
+        for (int q = 0; q < 5; ++q)
+            if (q < MS && is_reactant(n, j, q)) A[q] += aj;
     }
+    const float Wf = (float)W;
 #pragma unroll
     for (int s = 0; s < MS; ++s) {
         if (s >= n.S) break;
@@ -77,10 +81,11 @@ __device__ __forceinline__ void features_and_crit(const KernelArgs& a, const Net
 #pragma unroll
     for (int s = 0; s < 5; ++s) {
         if (s < n.S && s < MS) {
-            const double d = n.d[c * n.S + s];
-            const double As = A[s] + (d * (double)x[s < MS ? s : 0]) * W;
-            f[s] = __logf(1.0f + (float)x[s < MS ? s : 0]);
-            f[5 + s] = __logf(1.0f + (float)As);
+            const int sq = s < MS ? s : 0;
+            const float xf = (float)x[sq];
+            const float As = A[s] + ((float)n.d[c * n.S + s] * xf) * Wf;
+            f[s] = __logf(1.0f + xf);
+            f[5 + s] = __logf(1.0f + As);
         } else {
             f[s] = 0.0f;
             f[5 + s] = 0.0f;
@@ -94,11 +99,13 @@ __device__ __forceinline__ void features_and_crit(const KernelArgs& a, const Net
 
 // The proposals of one tile: features, the network on the tile, tau_v, the
 // exact tau'' clock of the given step, then min / argmin into the tile
-// partial.  Every lane of the wavefront takes part (the MLP is a wavefront
-// operation); act = the lane holds a subvolume.
+// partial.  Every lane of the wavefront takes part (the network runs on the
+// whole tile); act = the lane holds a subvolume; av / crit: react_props of
+// the lane's counts x.  W: the staged weights.
 template <int MS, int MR, int ACT>
 __device__ __forceinline__ void propose_tile(const KernelArgs& a, const NetView& n, const RealInfo& ri,
-                                             MlpLdsInfer& S, const uint32_t (&x)[MS], bool act, uint32_t v,
+                                             const MlpLdsW& W, const uint32_t (&x)[MS], const double (&av)[MR],
+                                             const uint32_t (&crit)[(MR + 31) / 32], bool act, uint32_t v,
                                              uint32_t step, double prev_tau, uint32_t t) {
     const MlpDev m{a.mlp_w32, (const __bf16*)a.mlp_q, a.mlp_norm};
     const float prev_logtau = prev_tau > 0.0 ? __logf((float)prev_tau) : 0.0f;
@@ -106,14 +113,13 @@ __device__ __forceinline__ void propose_tile(const KernelArgs& a, const NetView&
     double a0c = 0.0;
     bool live = false;
     if (act) {
-        features_and_crit<MS, MR>(a, n, ri, x, v, prev_logtau, f, a0c, live);
+        features_and_crit<MS, MR>(a, n, ri, x, av, crit, v, prev_logtau, f, a0c, live);
     } else {
 #pragma unroll
         for (int k = 0; k < 12; ++k) f[k] = 0.0f;
     }
-    stage_inputs(S.h, nullptr, m, f, act, kLdH);
     float o0, o1;
-    mlp_infer_tile<ACT>(S, m, o0, o1);
+    mlp_infer_regs<ACT>(W, m, f, act, o0, o1);
     double tp = kInf, t2 = kInf;
     uint32_t v2 = v;
     if (act) {
@@ -129,18 +135,37 @@ __device__ __forceinline__ void propose_tile(const KernelArgs& a, const NetView&
     }
     reduce_partA(tp, t2, v2);
     if ((threadIdx.x & 63u) == 0u) a.partA_tile[t] = PartA{tp, t2, v2, 0u};
-    wave_sync();
 }
 
-// Standalone learned tau pass: realizations in learned mode whose partials
-// do not hold proposals for the current state (first step, after a
-// rejection or leap-back, after the host changed the state or the network).
-template <int MS, int MR, int ACT>
+// The tau pass of a learned context, per tile: Cao's formula for a
+// realization whose last proposal was rejected (fallback), the network on
+// the current state where the partials do not hold its proposals (first
+// step, after a leap-back, after the host changed the state or the
+// network), nothing where the previous leap pass already proposed.  One
+// launch covers all three; in steady state every tile is skipped, and a
+// block with nothing to do leaves before staging the network.
+template <int MS, int MR, bool MC, int ACT>
 __global__ __launch_bounds__(256) void learned_tau_kernel(KernelArgs a) {
-    __shared__ MlpLdsInfer s_mlp[4];
-    const NetView n = net_view(a);
+    __shared__ MlpLdsW W;
+    __shared__ uint32_t s_need;
     const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t wv = wave_uniform(threadIdx.x >> 6);
+    if (threadIdx.x == 0) s_need = 0u;
+    __syncthreads();
+    for (uint32_t it = 0;; ++it) {
+        const uint32_t t = tile_at(it, a.ntiles, a.waves_per_block);
+        if (t == 0xFFFFFFFFu) break;
+        const cptr<RealCtl> cp = ro(a.ctl + wave_uniform(ld_ro(a.tiles + t).r));
+        if (!wave_uniform(cp->done) && !wave_uniform(cp->retry) &&
+            (wave_uniform(cp->fallback) || !wave_uniform(cp->prop_ok))) {
+            if (lane == 0) atomicOr(&s_need, 1u);
+            break;
+        }
+    }
+    __syncthreads();
+    if (!s_need) return;
+    stage_weights(W, MlpDev{a.mlp_w32, (const __bf16*)a.mlp_q, a.mlp_norm});
+    __syncthreads();
+    const NetView n = net_view<MC>(a);
     for (uint32_t it = 0;; ++it) {
         const uint32_t t = tile_at(it, a.ntiles, a.waves_per_block);
         if (t == 0xFFFFFFFFu) break;
@@ -148,29 +173,37 @@ __global__ __launch_bounds__(256) void learned_tau_kernel(KernelArgs a) {
         const uint32_t r = wave_uniform(te.r);
         const RealInfo ri = ld_ro(a.info + r);
         const cptr<RealCtl> cp = ro(a.ctl + r);
-        if (wave_uniform(cp->done) || wave_uniform(cp->retry) || wave_uniform(cp->fallback) ||
-            wave_uniform(cp->prop_ok))
+        if (wave_uniform(cp->done) || wave_uniform(cp->retry)) continue;
+        if (wave_uniform(cp->fallback)) {
+            cao_tau_tile<MS, MR>(a, n, ri, cp, te, t, false);
             continue;
+        }
+        if (wave_uniform(cp->prop_ok)) continue;
         const uint32_t* X = wave_uniform(cp->cur) ? a.x1 : a.x0;
         const uint32_t v = te.v0 + lane;
         const bool act = v < ri.N;
         uint32_t x[MS];
 #pragma unroll
         for (int s = 0; s < MS; ++s) x[s] = (act && s < n.S) ? X[ri.x_off + (uint64_t)s * ri.N + v] : 0u;
-        propose_tile<MS, MR, ACT>(a, n, ri, s_mlp[wv], x, act, v, wave_uniform(cp->step), cp->tau, t);
+        double av[MR];
+        uint32_t crit[(MR + 31) / 32];
+        react_props<MR, MS>(n, x, n.C > 1 && act ? a.comp[ri.comp_off + v] : 0, a.n_critical, av, crit);
+        propose_tile<MS, MR, ACT>(a, n, ri, W, x, av, crit, act, v, wave_uniform(cp->step), cp->tau, t);
     }
 }
 
 // End of the edge-parallel leap pass with the next step's proposals fused
-// in (learned contexts): finish_tile, then propose_tile on the new counts
-// for step + 1.  Proposals of a realization whose step is rejected are
-// never read (commit clears prop_ok).
+// in (learned contexts): finish_tile, the post-leap leap condition and the
+// network's proposals for step + 1 on the new counts, from one evaluation
+// of their propensities.  Proposals of a realization whose step is rejected
+// are never read (commit clears prop_ok).
 template <int MS, int MR, bool MC, int ACT>
 __global__ __launch_bounds__(256) void learned_finish_kernel(KernelArgs a) {
-    __shared__ MlpLdsInfer s_mlp[4];
+    __shared__ MlpLdsW W;
+    stage_weights(W, MlpDev{a.mlp_w32, (const __bf16*)a.mlp_q, a.mlp_norm});
+    __syncthreads();
     const NetView n = net_view<MC>(a);
     const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t wv = wave_uniform(threadIdx.x >> 6);
     for (uint32_t it = 0;; ++it) {
         const uint32_t t = tile_at(it, a.ntiles, a.waves_per_block);
         if (t == 0xFFFFFFFFu) break;
@@ -185,14 +218,25 @@ __global__ __launch_bounds__(256) void learned_finish_kernel(KernelArgs a) {
         uint32_t bad = 0, viol = 0;
         uint32_t x[MS];
         long long nx[MS];
-        finish_tile<MS, MR>(a, n, ri, cur ? a.x1 : a.x0, cur ? a.x0 : a.x1, te.v0, sv, fire, bad, viol, x, nx);
-        reduce_partB(fire, bad, viol);
-        if (lane == 0) a.partB_tile[t] = PartB{a.partB_tile[t].fire + fire, bad, viol};
+        finish_tile<MS, MR>(a, n, ri, cur ? a.x1 : a.x0, cur ? a.x0 : a.x1, te.v0, sv, fire, bad, viol, x, nx, false);
         const uint32_t v = te.v0 + lane;
+        const bool act = v < ri.N;
+        const int c = n.C > 1 && act ? a.comp[ri.comp_off + v] : 0;
         uint32_t xn[MS];
 #pragma unroll
         for (int s = 0; s < MS; ++s) xn[s] = (uint32_t)(nx[s] < 0 ? 0 : nx[s]);
-        propose_tile<MS, MR, ACT>(a, n, ri, s_mlp[wv], xn, v < ri.N, v, sv.step + 1u, sv.tau, t);
+        double av1[MR];
+        uint32_t crit1[(MR + 31) / 32];
+        react_props<MR, MS>(n, xn, c, a.n_critical, av1, crit1);
+        if (act) {
+            double av0[MR];
+            uint32_t crit0[(MR + 31) / 32];
+            react_props<MR, MS>(n, x, c, a.n_critical, av0, crit0);
+            viol = leap_condition_broken<MS, MR>(n, c, a.n_critical, a.epsilon, a.theta, x, nx, av0, crit0, av1);
+        }
+        reduce_partB(fire, bad, viol);
+        if (lane == 0) a.partB_tile[t] = PartB{a.partB_tile[t].fire + fire, bad, viol};
+        propose_tile<MS, MR, ACT>(a, n, ri, W, xn, av1, crit1, act, v, sv.step + 1u, sv.tau, t);
     }
 }
 
@@ -228,7 +272,12 @@ __global__ __launch_bounds__(256) void records_kernel(KernelArgs a) {
         float f[12];
         double a0c, tp, a0c2;
         bool live;
-        features_and_crit<MS, MR>(a, n, ri, x, v, prev_logtau, f, a0c, live);
+        {
+            double av[MR];
+            uint32_t crit[(MR + 31) / 32];
+            react_props<MR, MS>(n, x, n.C > 1 ? a.comp[ri.comp_off + v] : 0, a.n_critical, av, crit);
+            features_and_crit<MS, MR>(a, n, ri, x, av, crit, v, prev_logtau, f, a0c, live);
+        }
         tau_subvolume<MS, MR>(a, n, ri, X, v, tp, a0c2);
 #pragma unroll
         for (int k = 0; k < 12; ++k) a.rec_feats[(size_t)v * 12 + k] = f[k];
@@ -295,8 +344,10 @@ __global__ __launch_bounds__(256) void records_kernel(KernelArgs a) {
 }
 
 #define PLIS_INST_L(MS, MR)                                                          \
-    template __global__ void learned_tau_kernel<MS, MR, 0>(KernelArgs);             \
-    template __global__ void learned_tau_kernel<MS, MR, 1>(KernelArgs);             \
+    template __global__ void learned_tau_kernel<MS, MR, false, 0>(KernelArgs);      \
+    template __global__ void learned_tau_kernel<MS, MR, false, 1>(KernelArgs);      \
+    template __global__ void learned_tau_kernel<MS, MR, true, 0>(KernelArgs);       \
+    template __global__ void learned_tau_kernel<MS, MR, true, 1>(KernelArgs);       \
     template __global__ void learned_finish_kernel<MS, MR, false, 0>(KernelArgs);   \
     template __global__ void learned_finish_kernel<MS, MR, false, 1>(KernelArgs);   \
     template __global__ void learned_finish_kernel<MS, MR, true, 0>(KernelArgs);    \

@@ -108,13 +108,6 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// LDS of the inference-only path (one wavefront): one buffer of 64 rows
-// holds the standardised inputs (columns 0..15) and then each hidden layer's
-// activations, every layer reading a tile half before overwriting it.
-struct MlpLdsInfer {
-    __bf16 h[64 * kLdH];
-};
-
 // Standardise lane's feature row into LDS row r (stride ld; lane r owns row r).
 __device__ __forceinline__ void stage_inputs(__bf16* x, __bf16* xt, const MlpDev& m, const float (&f)[12],
                                              bool valid, int ld = kLdX) {
@@ -410,95 +403,165 @@ __global__ __launch_bounds__(256) void mlp_pack_kernel(const float* w32, __bf16*
     q[e] = (__bf16)v;
 }
 
-// One hidden layer of the inference tile in transposed form,
-// out^T = relu(W^T in^T + b): the A operand is the packed W^T (rows = units),
-// the B operand the tile rows of `in`, so lane l's accumulator holds four
-// consecutive units of one tile row and stores them with one 8-byte write
-// into the row-major activations the next layer reads.  The two halves of
-// the tile (rows 0..31, 32..63) go one after the other; a half's B
-// fragments are read before its rows are written, so `out` may be `in`.
-template <int K, int ACT>
-__device__ __forceinline__ void infer_layer(const __bf16* in, int ldin, const __bf16* wt, const float* bias,
-                                            __bf16* out) {
-    typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
-    const int l = threadIdx.x & 63;
-#pragma unroll 1
-    for (int nb = 0; nb < 2; ++nb) {
-        bf16x8 bfr[K / 16];
+// ---------------------------------------------------------------------------
+// inference: the three layers chained in registers
+//
+// Each layer runs in transposed form, out^T = W^T in^T: the A operand is a
+// fragment of W^T (staged in LDS once per block), the B operand the tile's
+// activations.  A lane's accumulator holds units 8(i/4) + 4(l/32) + i%4 of
+// tile row l%32 (rows 32..63 in the second half-tile nb), and the next
+// layer's B fragment wants 8 consecutive units of that row: lane l < 32
+// already holds half of them, its partner l + 32 the other half, so one
+// v_permlane32_swap per packed bf16 pair builds the fragment without any
+// LDS round trip.  Input rows enter the same way (lane r holds row r).
+// ---------------------------------------------------------------------------
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+constexpr int kLdW1 = 24, kLdW2 = 72;  // LDS row strides (bf16), padded against bank conflicts
+
+// Weights of the inference path in LDS: W1^T [64][16], W2^T [64][64],
+// W3^T [32][64] (rows 2..31 zero), biases.
+struct MlpLdsW {
+    __bf16 w1t[kNh * kLdW1];
+    __bf16 w2t[kNh * kLdW2];
+    __bf16 w3t[kNoPad * kLdW2];
+    float b1[kNh], b2[kNh], b3[4];
+};
+
+// Whole block, 16-byte copies; the caller synchronises the block after it.
+__device__ __forceinline__ void stage_weights(MlpLdsW& W, const MlpDev& m) {
+    const uint32_t nt = blockDim.x;
+    const uint4* q = reinterpret_cast<const uint4*>(m.q);
+    for (uint32_t e = threadIdx.x; e < (uint32_t)kNh * 2u; e += nt)
+        *reinterpret_cast<uint4*>(&W.w1t[(e >> 1) * kLdW1 + (e & 1u) * 8u]) = q[(kQW1t >> 3) + e];
+    for (uint32_t e = threadIdx.x; e < (uint32_t)kNh * 8u; e += nt)
+        *reinterpret_cast<uint4*>(&W.w2t[(e >> 3) * kLdW2 + (e & 7u) * 8u]) = q[(kQW2t >> 3) + e];
+    for (uint32_t e = threadIdx.x; e < (uint32_t)kNoPad * 8u; e += nt)
+        *reinterpret_cast<uint4*>(&W.w3t[(e >> 3) * kLdW2 + (e & 7u) * 8u]) = q[(kQW3t >> 3) + e];
+    for (uint32_t e = threadIdx.x; e < (uint32_t)kNh; e += nt) {
+        W.b1[e] = m.w32[kPB1 + e];
+        W.b2[e] = m.w32[kPB2 + e];
+    }
+    if (threadIdx.x < (uint32_t)kNo) W.b3[threadIdx.x] = m.w32[kPB3 + threadIdx.x];
+}
+
+// The dword of lane l ^ 32 (v_permlane32_swap exchanges lanes 32..63 of its
+// first operand with lanes 0..31 of its second).
+__device__ __forceinline__ uint32_t xhalf(uint32_t v) {
+    const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    return (threadIdx.x & 32u) ? r[0] : r[1];
+}
+
+__device__ __forceinline__ uint32_t pack_bf16(float lo, float hi) {
+    const bf16x2 v = {(__bf16)lo, (__bf16)hi};
+    return __builtin_bit_cast(uint32_t, v);
+}
+
+// act(acc + bias) of one 32-unit accumulator block as 8 packed bf16 pairs
+// (pair j = elements 2j, 2j + 1); bias points at the block's first unit.
+template <int ACT>
+__device__ __forceinline__ void act_pack(const f32x16& c, const float* bias, uint32_t (&P)[8]) {
+    const uint32_t hi = (threadIdx.x >> 5) & 1u;
 #pragma unroll
-        for (int kb = 0; kb < K / 16; ++kb) bfr[kb] = lds_frag(in, ldin, nb * 32, kb * 16);
-        wave_sync();
-        const int row = nb * 32 + (l & 31);
-#pragma unroll 1
+    for (int j = 0; j < 4; ++j) {
+        const f32x4 b = *reinterpret_cast<const f32x4*>(bias + 8 * j + 4 * (int)hi);
+        P[2 * j] = pack_bf16(act_f<ACT>(c[4 * j] + b.x), act_f<ACT>(c[4 * j + 1] + b.y));
+        P[2 * j + 1] = pack_bf16(act_f<ACT>(c[4 * j + 2] + b.z), act_f<ACT>(c[4 * j + 3] + b.w));
+    }
+}
+
+// B fragment of the next layer for k block 2 mb + H from the packed
+// activations P of accumulator block mb: lane l < 32 keeps units 16H + 0..3
+// and receives 16H + 4..7 from its partner; lane l >= 32 receives
+// 16H + 8..11 and keeps 16H + 12..15.
+template <int H>
+__device__ __forceinline__ bf16x8 next_frag(const uint32_t (&P)[8]) {
+    const bool up = (threadIdx.x & 32u) != 0u;
+    const uint32_t r0 = xhalf(up ? P[4 * H] : P[4 * H + 2]);
+    const uint32_t r1 = xhalf(up ? P[4 * H + 1] : P[4 * H + 3]);
+    const u32x4 f = up ? u32x4{r0, r1, P[4 * H + 2], P[4 * H + 3]} : u32x4{P[4 * H], P[4 * H + 1], r0, r1};
+    return __builtin_bit_cast(bf16x8, f);
+}
+
+// Forward pass of one 64-row tile: lane r holds feature row r (valid: the
+// row exists); returns row r's two outputs.  W holds the staged weights.
+template <int ACT>
+__device__ __forceinline__ void mlp_infer_regs(const MlpLdsW& W, const MlpDev& m, const float (&f)[12], bool valid,
+                                               float& o0, float& o1) {
+    const bool up = (threadIdx.x & 32u) != 0u;
+    uint32_t q[8];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        const float z0 = valid ? (f[2 * j] - m.norm[2 * j]) * (1.0f / m.norm[12 + 2 * j]) : 0.0f;
+        const float z1 = valid ? (f[2 * j + 1] - m.norm[2 * j + 1]) * (1.0f / m.norm[13 + 2 * j]) : 0.0f;
+        q[j] = pack_bf16(z0, z1);
+    }
+    q[6] = 0u;
+    q[7] = 0u;
+    // layer-1 B fragments: features 0..7 / 8..15 of rows 0..31 (nb 0) and 32..63 (nb 1)
+    uint32_t r[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) r[i] = xhalf(up ? q[i] : q[4 + i]);
+    bf16x8 B1[2];
+    B1[0] = __builtin_bit_cast(bf16x8, up ? u32x4{r[0], r[1], r[2], r[3]} : u32x4{q[0], q[1], q[2], q[3]});
+    B1[1] = __builtin_bit_cast(bf16x8, up ? u32x4{q[4], q[5], q[6], q[7]} : u32x4{r[0], r[1], r[2], r[3]});
+    float out[2][2];
+#pragma unroll
+    for (int nb = 0; nb < 2; ++nb) {
+        bf16x8 B2[4];
+#pragma unroll
+        for (int mb = 0; mb < 2; ++mb) {
+            const f32x16 c = mfma(lds_frag(W.w1t, kLdW1, mb * 32, 0), B1[nb], f32x16{});
+            uint32_t P[8];
+            act_pack<ACT>(c, W.b1 + mb * 32, P);
+            B2[2 * mb] = next_frag<0>(P);
+            B2[2 * mb + 1] = next_frag<1>(P);
+        }
+        bf16x8 B3[4];
+#pragma unroll
         for (int mb = 0; mb < 2; ++mb) {
             f32x16 c = {};
 #pragma unroll
-            for (int kb = 0; kb < K / 16; ++kb) c = mfma(glb_frag(wt, K, mb * 32, kb * 16), bfr[kb], c);
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int u0 = mb * 32 + 8 * j + 4 * (l >> 5);
-                const float4 bb = *reinterpret_cast<const float4*>(bias + u0);
-                bf16x4 v;
-                v[0] = (__bf16)act_f<ACT>(c[4 * j + 0] + bb.x);
-                v[1] = (__bf16)act_f<ACT>(c[4 * j + 1] + bb.y);
-                v[2] = (__bf16)act_f<ACT>(c[4 * j + 2] + bb.z);
-                v[3] = (__bf16)act_f<ACT>(c[4 * j + 3] + bb.w);
-                *reinterpret_cast<bf16x4*>(out + row * kLdH + u0) = v;
-            }
+            for (int kb = 0; kb < 4; ++kb) c = mfma(lds_frag(W.w2t, kLdW2, mb * 32, kb * 16), B2[kb], c);
+            uint32_t P[8];
+            act_pack<ACT>(c, W.b2 + mb * 32, P);
+            B3[2 * mb] = next_frag<0>(P);
+            B3[2 * mb + 1] = next_frag<1>(P);
         }
-        wave_sync();
-    }
-}
-
-// Forward pass of one 64-row tile for inference (inputs staged in s.h);
-// returns lane r's two outputs (row r).  The output layer is W3^T (padded
-// to 32 rows) times the tile: lanes 0..31 receive rows l and 32 + l.
-template <int ACT>
-__device__ __forceinline__ void mlp_infer_tile(MlpLdsInfer& s, const MlpDev& m, float& o0, float& o1) {
-    wave_sync();
-    infer_layer<kNin, ACT>(s.h, kLdH, m.q + kQW1t, m.w32 + kPB1, s.h);
-    wave_sync();
-    infer_layer<kNh, ACT>(s.h, kLdH, m.q + kQW2t, m.w32 + kPB2, s.h);
-    wave_sync();
-    const int l = threadIdx.x & 63;
-    float a0 = 0.0f, a1 = 0.0f, b0 = 0.0f, b1 = 0.0f;
-#pragma unroll 1
-    for (int nb = 0; nb < 2; ++nb) {
         f32x16 c = {};
 #pragma unroll
-        for (int kb = 0; kb < kNh / 16; ++kb)
-            c = mfma(glb_frag(m.q + kQW3t, kNh, 0, kb * 16), lds_frag(s.h, kLdH, nb * 32, kb * 16), c);
-        if (nb == 0) {
-            a0 = c[0];
-            a1 = c[1];
-        } else {
-            b0 = c[0];
-            b1 = c[1];
-        }
+        for (int kb = 0; kb < 4; ++kb) c = mfma(lds_frag(W.w3t, kLdW2, 0, kb * 16), B3[kb], c);
+        // units 0 and 1 of row nb 32 + l % 32 sit in lanes l < 32
+        out[nb][0] = c[0];
+        out[nb][1] = c[1];
     }
-    const float t0 = __shfl(b0, (l + 32) & 63), t1 = __shfl(b1, (l + 32) & 63);
-    o0 = (l < 32 ? a0 : t0) + m.w32[kPB3 + 0];
-    o1 = (l < 32 ? a1 : t1) + m.w32[kPB3 + 1];
+    const float t0 = __builtin_bit_cast(float, xhalf(__builtin_bit_cast(uint32_t, out[1][0])));
+    const float t1 = __builtin_bit_cast(float, xhalf(__builtin_bit_cast(uint32_t, out[1][1])));
+    o0 = (up ? t0 : out[0][0]) + W.b3[0];
+    o1 = (up ? t1 : out[0][1]) + W.b3[1];
 }
 
-// Inference over given feature rows (parity checks): out[n][2].
+// Inference over given feature rows (parity checks and the host API):
+// out[n][2].  One wavefront per block, grid-stride over tiles.
 template <int ACT>
 __global__ __launch_bounds__(64) void mlp_infer_kernel(MlpDev m, const float* feats, uint32_t n, float* out) {
-    __shared__ MlpLdsInfer s;
+    __shared__ MlpLdsW W;
+    stage_weights(W, m);
+    __syncthreads();
     const int l = threadIdx.x & 63;
     for (uint32_t t = blockIdx.x; t * 64u < n; t += gridDim.x) {
         const uint32_t r = t * 64u + (uint32_t)l;
         float f[12];
 #pragma unroll
         for (int k = 0; k < 12; ++k) f[k] = r < n ? feats[(size_t)r * 12 + k] : 0.f;
-        stage_inputs(s.h, nullptr, m, f, r < n, kLdH);
         float o0, o1;
-        mlp_infer_tile<ACT>(s, m, o0, o1);
+        mlp_infer_regs<ACT>(W, m, f, r < n, o0, o1);
         if (r < n) {
             out[(size_t)r * 2] = o0;
             out[(size_t)r * 2 + 1] = o1;
         }
-        wave_sync();
     }
 }
 

@@ -297,6 +297,19 @@ __device__ __forceinline__ int poisson_fast_f(float lf, uint32_t u0) {
     return poisson_squeeze(lf, u);
 }
 
+// The zero shortcuts alone: 0 when the draw is certainly 0 (the value
+// poisson() returns), -1 when it may not be.  poisson_zero_f takes a float
+// mean (as poisson_fast_f), poisson_zero a double one (as poisson()).
+__device__ __forceinline__ int poisson_zero_f(float lf, uint32_t u0) {
+    if (!(lf > 0.0f)) return 0;
+    const float u = __uint2float_rn(u0) * 0x1p-32f;
+    return u <= (1.0f - lf) - 0x1p-19f ? 0 : -1;
+}
+__device__ __forceinline__ int poisson_zero(double lam, uint32_t u0) {
+    if (!(lam > 0.0)) return 0;
+    return uniform01(u0) <= (1.0 - lam) - 0x1p-50 ? 0 : -1;
+}
+
 // The fast part of poisson() for kernels that defer the rare hard cases:
 // the zero shortcut and the float squeeze settle the draw (k >= 0, the value
 // poisson() returns); -1 leaves it to the double search (inversion_search)

@@ -20,12 +20,13 @@ pytestmark = pytest.mark.gpu
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CASES = os.path.join(HERE, "golden", "ref_cases")
+BENCH = os.path.join(HERE, "golden", "ref_benchmarks")
 DRIVER = os.path.join(os.path.dirname(HERE), "pliscuda_amd", "bin", "pliscuda")
 SEED = 4321
 
 
-def run_case(name, tmp_path, extra=""):
-    src = os.path.join(CASES, name)
+def run_case(name, tmp_path, extra="", root=CASES):
+    src = os.path.join(root, name)
     for f in os.listdir(src):
         if f.endswith(".txt"):
             shutil.copyfile(os.path.join(src, f), tmp_path / f)
@@ -176,3 +177,25 @@ def test_realizations_with_log_freq_all_reach_end_time(tmp_path):
         xo = c["x0"].copy()
         ffi.run(model, xo, 0.05, 10, SEED, c["t_end"], realization=q)
         assert np.array_equal(x, xo.astype(np.int64)), q
+
+
+@pytest.mark.parametrize("name", ["diff-64", "react-64", "chemotaxis", "diff-5125"])
+def test_reference_benchmark_inputs(name, tmp_path):
+    """The reference's benchmark inputs (PLISCuda/benchmarks/*, the cases its
+    benchmarks/run.sh times) through the driver: the run reaches endTime,
+    reports steps/second and subvolume-leaps/second, keeps every
+    stoichiometric conservation law, and matches the oracle bit for bit
+    (diff-5125, 200 k pure-tau'' steps: exact mass and a spread front)."""
+    out, t, x, final, _ = run_case(name, tmp_path, root=BENCH)
+    c = inputs.read_conf(os.path.join(BENCH, name, "conf.txt"))
+    assert t == pytest.approx(c["t_end"], abs=1e-6)
+    for key in ("steps/second:", "subvolume-leaps/second:", "firings:"):
+        assert key in out
+    assert conservation_ok(c["net"], c["x0"].astype(np.int64), x.ravel())
+    if name == "diff-5125":
+        assert int(x.sum()) == 100000 and (x > 0).sum() > 200
+        return
+    model = ffi.Model(c["net"], c["topo"])
+    xo = c["x0"].copy()
+    ffi.run(model, xo, 0.05, 10, SEED, c["t_end"])
+    assert np.array_equal(x.ravel(), xo.astype(np.int64)), name
