@@ -111,25 +111,23 @@ __device__ __forceinline__ uint32_t tile_at(uint32_t it, uint32_t ntiles, uint32
 }
 
 __device__ __forceinline__ double binom_d(uint32_t x, int r) {
-    if ((long long)x < (long long)r) return 0.0;
+    if (x < (uint32_t)r) return 0.0;  // r >= 1
     const double xd = (double)x;
     if (r == 1) return xd;
     if (r == 2) return (xd * (xd - 1.0)) * 0.5;
     return ((xd * (xd - 1.0)) * (xd - 2.0)) / 6.0;
 }
 
-// floor(x / r) for a reactant coefficient r (1..3 for orders up to 3, any r
-// handled): shifts and a division by the constant 3 instead of a runtime
-// 32-bit division.
-__device__ __forceinline__ uint32_t div_coef(uint32_t x, int r) {
-    if (r == 1) return x;
-    if (r == 2) return x >> 1;
-    if (r == 3) return x / 3u;
-    return x / (uint32_t)r;
+// Cao's critical test of one reactant, floor(x / r) < n_c, as the
+// equivalent x < n_c r (integers, r >= 1, n_c >= 0): no division; n_c r is
+// formed in 64 bits from scalars, one 64-bit compare per lane.
+__device__ __forceinline__ bool below_nc(uint32_t x, int r, int nc) {
+    return (uint64_t)x < (uint64_t)(uint32_t)nc * (uint64_t)(uint32_t)r;
 }
 
+// A hop is active (non-critical) when d > 0 and x >= max(n_c, 1).
 __device__ __forceinline__ bool hop_active(double d, uint32_t x, int nc) {
-    return d > 0.0 && (long long)x >= (long long)nc && x > 0u;
+    return d > 0.0 && x >= (uint32_t)(nc > 1 ? nc : 1);
 }
 
 __device__ __forceinline__ double g_cao(int hor, int mult, uint32_t xs) {
@@ -152,7 +150,8 @@ __device__ __forceinline__ double eps_x_over_g(double eps, int hor, int mult, ui
 }
 
 // Reaction propensities a_j = k_j prod_s C(x_s, r_sj) and the critical flags
-// (Cao: L_j = min_s floor(x_s / r_sj) < n_c) at one subvolume.
+// (Cao: L_j = min_s floor(x_s / r_sj) < n_c, i.e. some reactant has
+// floor(x_s / r_sj) < n_c) at one subvolume.
 template <int MR, int MS>
 __device__ __forceinline__ void react_props(const NetView& n, const uint32_t (&x)[MS], int c, int nc,
                                             double (&av)[MR], uint32_t (&crit)[(MR + 31) / 32]) {
@@ -162,7 +161,7 @@ __device__ __forceinline__ void react_props(const NetView& n, const uint32_t (&x
     for (int j = 0; j < MR; ++j) {
         if (j >= n.R) break;
         double aj = n.k[c * n.R + j];
-        long long L = -1;
+        bool low = false;
         const int nr = n.nrs[j];
         for (int i = 0; i < nr; ++i) {
             const int s = n.rs[j * 3 + i], r = n.rr[j * 3 + i];
@@ -171,11 +170,10 @@ __device__ __forceinline__ void react_props(const NetView& n, const uint32_t (&x
             for (int q = 0; q < MS; ++q)
                 if (q == s) xs = x[q];
             aj = aj * binom_d(xs, r);
-            const long long l = (long long)div_coef(xs, r);
-            if (L < 0 || l < L) L = l;
+            low = low || below_nc(xs, r, nc);
         }
         av[j] = aj;
-        if (n.order[j] > 0 && L < (long long)nc) crit[j >> 5] |= 1u << (j & 31);
+        if (n.order[j] > 0 && low) crit[j >> 5] |= 1u << (j & 31);
     }
 }
 
@@ -1029,16 +1027,15 @@ __device__ __forceinline__ void walk_fwd_draws(const KernelArgs& a, const NetVie
     const uint32_t c3base = ctr_word3(sv.real, sv.attempt, kReact, 0u);
     for (int j = 0; j < n.R; ++j) {
         double aj = n.k[c * n.R + j];
-        long long L_j = -1;
+        bool low = false;
         const int nr = n.nrs[j];
         for (int q = 0; q < nr; ++q) {
             const int sq = n.rs[j * 3 + q], rq = n.rr[j * 3 + q];
             const uint32_t xs = s_x[sq][lane];
             aj = aj * binom_d(xs, rq);
-            const long long l = (long long)div_coef(xs, rq);
-            if (L_j < 0 || l < L_j) L_j = l;
+            low = low || below_nc(xs, rq, nc);
         }
-        if ((n.order[j] > 0 && L_j < (long long)nc) || !(aj > 0.0)) continue;
+        if ((n.order[j] > 0 && low) || !(aj > 0.0)) continue;
         const uint32_t b = (uint32_t)j >> 2;
         if (b != blk) {
             o = philox10(v, kSelfId, sv.step, c3base | b, k0, k1);
