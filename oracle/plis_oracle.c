@@ -712,6 +712,30 @@ static void decide(ctl_t *st)
     st->tau = tau;
 }
 
+/* Uniform word of leap draw w (hop: w = 2s + direction of edge {c0 < c1},
+ * direction 0 for the hop c0 -> c1; reaction: w = j, c1 = SELF_ID): its top
+ * byte is byte w & 15 (bytes 0..3 of word 0, then words 1..3) of the coarse
+ * block c3base | (w >> 4), its low 24 bits word w & 3 of the fine block
+ * c3base | (FINE_BLOCK + (w >> 2)), whose counter fctr receives (the PTRS
+ * stream's).  The kernels settle a draw from its top byte alone when every
+ * word with that byte gives 0 (plis_numerics.h, coarse_zero); the
+ * draws themselves are the inversion / PTRS of this word. */
+#define FINE_BLOCK 64u
+static uint32_t leap_word(uint32_t c0, uint32_t c1, uint32_t step, uint32_t c3base, const uint32_t key[2],
+                          uint32_t w, uint32_t fctr[4])
+{
+    uint32_t cc[4] = {c0, c1, step, c3base | (w >> 4)};
+    uint32_t o[4];
+    orc_philox4x32_10(cc, key, o);
+    uint32_t hb = (o[(w >> 2) & 3u] >> (8u * (w & 3u))) & 0xFFu;
+    fctr[0] = c0;
+    fctr[1] = c1;
+    fctr[2] = step;
+    fctr[3] = c3base | (FINE_BLOCK + (w >> 2));
+    orc_philox4x32_10(fctr, key, o);
+    return (hb << 24) | (o[w & 3u] & 0xFFFFFFu);
+}
+
 /* one leap at v; returns 1 on a negative (or overflowing) population */
 static int leap_v(const model_t *m, const csr_t *g, const orc_params *p, const uint32_t *x,
                   uint32_t *xn, uint32_t v, uint32_t real, const ctl_t *st, uint64_t *fire)
@@ -729,10 +753,9 @@ static int leap_v(const model_t *m, const csr_t *g, const orc_params *p, const u
     /* non-critical reactions */
     for (int j = 0; j < m->R; j++) {
         if (crit[j] || !(a[j] > 0.0)) continue;
-        uint32_t cc[4] = {v, SELF_ID, st->step, ctr3(real, st->attempt, P_REACT, (uint32_t)j >> 2)};
-        uint32_t o[4];
-        orc_philox4x32_10(cc, key, o);
-        int64_t k = orc_poisson(tau * a[j], o[j & 3], cc, key, (uint32_t)j & 3u);
+        uint32_t fc[4];
+        uint32_t wd = leap_word(v, SELF_ID, st->step, ctr3(real, st->attempt, P_REACT, 0), key, (uint32_t)j, fc);
+        int64_t k = orc_poisson(tau * a[j], wd, fc, key, (uint32_t)j & 3u);
         f += (uint64_t)k;
         for (int s = 0; s < m->S; s++) nx[s] += k * (int64_t)m->nu[s * m->R + j];
     }
@@ -744,13 +767,12 @@ static int leap_v(const model_t *m, const csr_t *g, const orc_params *p, const u
         double dx = d * (double)xs;
         for (uint32_t q = g->rp[v]; q < g->rp[v + 1]; q++) {
             uint32_t u = g->col[q];
-            uint32_t wd = 2u * (uint32_t)s + (v > u ? 1u : 0u);
-            uint32_t cc[4] = {v < u ? v : u, v < u ? u : v, st->step,
-                              ctr3(real, st->attempt, P_HOP, wd >> 2)};
-            uint32_t o[4];
-            orc_philox4x32_10(cc, key, o);
+            uint32_t w = 2u * (uint32_t)s + (v > u ? 1u : 0u);
+            uint32_t fc[4];
+            uint32_t wd = leap_word(v < u ? v : u, v < u ? u : v, st->step, ctr3(real, st->attempt, P_HOP, 0),
+                                    key, w, fc);
             double lam = tau * (dx * (double)g->w[q]);
-            int64_t k = orc_poisson(lam, o[wd & 3u], cc, key, wd & 3u);
+            int64_t k = orc_poisson(lam, wd, fc, key, w & 3u);
             nx[s] -= k;
             f += (uint64_t)k;
         }
@@ -762,13 +784,12 @@ static int leap_v(const model_t *m, const csr_t *g, const orc_params *p, const u
             uint32_t xu = X(x, N, s, u);
             double du = m->d[comp_of(g, u) * m->S + s];
             if (!hop_active(m, p, du, xu)) continue;
-            uint32_t wd = 2u * (uint32_t)s + (u > v ? 1u : 0u);
-            uint32_t cc[4] = {v < u ? v : u, v < u ? u : v, st->step,
-                              ctr3(real, st->attempt, P_HOP, wd >> 2)};
-            uint32_t o[4];
-            orc_philox4x32_10(cc, key, o);
+            uint32_t w = 2u * (uint32_t)s + (u > v ? 1u : 0u);
+            uint32_t fc[4];
+            uint32_t wd = leap_word(v < u ? v : u, v < u ? u : v, st->step, ctr3(real, st->attempt, P_HOP, 0),
+                                    key, w, fc);
             double lam = tau * ((du * (double)xu) * (double)g->iw[q]);
-            int64_t k = orc_poisson(lam, o[wd & 3u], cc, key, wd & 3u);
+            int64_t k = orc_poisson(lam, wd, fc, key, w & 3u);
             nx[s] += k;
         }
     }

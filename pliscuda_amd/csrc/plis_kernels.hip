@@ -686,14 +686,13 @@ __device__ __forceinline__ bool leap_subvolume(const KernelArgs& a, const NetVie
         const uint32_t c3base = ctr_word3(sv.real, sv.attempt, kReact, 0u);
         for (int j = 0; j < n.R; ++j) {
             if (is_crit<MR>(crit, j) || !(av[j] > 0.0)) continue;
-            const uint32_t b = (uint32_t)j >> 2;
+            const uint32_t b = (uint32_t)j >> 4;
             if (b != blk) {
                 o = philox10(v, kSelfId, sv.step, c3base | b, k0, k1);
                 blk = b;
             }
-            const uint32_t wd = (uint32_t)j & 3u;
-            const DrawId id{v, kSelfId, sv.step, c3base | b, k0, k1, wd};
-            const long long k = poisson(tau * av[j], pick(o, wd), id);
+            const long long k = leap_draw(tau * av[j], coarse_byte(o, (uint32_t)j), v, kSelfId, sv.step, c3base,
+                                          k0, k1, (uint32_t)j);
             fire += (unsigned long long)k;
 #pragma unroll
             for (int s = 0; s < MS; ++s)
@@ -714,7 +713,7 @@ __device__ __forceinline__ bool leap_subvolume(const KernelArgs& a, const NetVie
     }
     const uint32_t c3hop = ctr_word3(sv.real, sv.attempt, kHop, 0u);
     if (sym) {
-        // One Philox block per (edge, species pair) serves both directions of
+        // One coarse block per (edge, species pair) serves both directions of
         // both species.  Pairs outer (unrolled: static indices keep the count
         // accumulators in registers), edges inner (rolled: few live values).
         const uint32_t* colp = a.col + ri.e_off;
@@ -727,7 +726,8 @@ __device__ __forceinline__ bool leap_subvolume(const KernelArgs& a, const NetVie
             const double dx0 = dx[s0], dx1 = two ? dx[s0 + 1 < MS ? s0 + 1 : s0] : 0.0;
             const uint32_t* X0 = X + ri.x_off + (uint64_t)s0 * N;
             const uint32_t* X1 = two ? X0 + N : X0;
-            const uint32_t cb = c3hop | ((uint32_t)s0 >> 1);
+            const uint32_t cb = c3hop | ((uint32_t)s0 >> 3);
+            const uint32_t w0 = 2u * (uint32_t)s0;
             long long d0 = 0, d1 = 0;
             for (uint32_t q = rb; q < re; ++q) {
                 const uint32_t u = colp[q];
@@ -745,18 +745,21 @@ __device__ __forceinline__ bool leap_subvolume(const KernelArgs& a, const NetVie
                 const uint32_t lo = v < u ? v : u, hi = v < u ? u : v;
                 const uint32_t dout = v > u ? 1u : 0u, din = 1u - dout;
                 const U4 o = philox10(lo, hi, sv.step, cb, k0, k1);
+                auto hop = [&](double l, uint32_t w) {
+                    return leap_draw(l, coarse_byte(o, w), lo, hi, sv.step, c3hop, k0, k1, w);
+                };
                 if (lo0 > 0.0) {
-                    const long long k = poisson(lo0, pick(o, dout), DrawId{lo, hi, sv.step, cb, k0, k1, dout});
+                    const long long k = hop(lo0, w0 + dout);
                     d0 -= k;
                     fire += (unsigned long long)k;
                 }
-                if (li0 > 0.0) d0 += poisson(li0, pick(o, din), DrawId{lo, hi, sv.step, cb, k0, k1, din});
+                if (li0 > 0.0) d0 += hop(li0, w0 + din);
                 if (lo1 > 0.0) {
-                    const long long k = poisson(lo1, pick(o, 2u + dout), DrawId{lo, hi, sv.step, cb, k0, k1, 2u + dout});
+                    const long long k = hop(lo1, w0 + 2u + dout);
                     d1 -= k;
                     fire += (unsigned long long)k;
                 }
-                if (li1 > 0.0) d1 += poisson(li1, pick(o, 2u + din), DrawId{lo, hi, sv.step, cb, k0, k1, 2u + din});
+                if (li1 > 0.0) d1 += hop(li1, w0 + 2u + din);
             }
 #pragma unroll
             for (int q = 0; q < MS; ++q) {
@@ -775,9 +778,9 @@ __device__ __forceinline__ bool leap_subvolume(const KernelArgs& a, const NetVie
                 if (s >= n.S) break;
                 if (!act[s]) continue;
                 const uint32_t wd = 2u * (uint32_t)s + dout;
-                const U4 o = philox10(lo, hi, sv.step, c3hop | (wd >> 2), k0, k1);
-                const DrawId id{lo, hi, sv.step, c3hop | (wd >> 2), k0, k1, wd & 3u};
-                const long long k = poisson(tau * (dx[s] * wq), pick(o, wd & 3u), id);
+                const U4 o = philox10(lo, hi, sv.step, c3hop | (wd >> 4), k0, k1);
+                const long long k =
+                    leap_draw(tau * (dx[s] * wq), coarse_byte(o, wd), lo, hi, sv.step, c3hop, k0, k1, wd);
                 nx[s] -= k;
                 fire += (unsigned long long)k;
             }
@@ -796,9 +799,9 @@ __device__ __forceinline__ bool leap_subvolume(const KernelArgs& a, const NetVie
                 const double du = n.d[cu * n.S + s];
                 if (!hop_active(du, xu, nc)) continue;
                 const uint32_t wd = 2u * (uint32_t)s + din;
-                const U4 o = philox10(lo, hi, sv.step, c3hop | (wd >> 2), k0, k1);
-                const DrawId id{lo, hi, sv.step, c3hop | (wd >> 2), k0, k1, wd & 3u};
-                nx[s] += poisson(tau * ((du * (double)xu) * wq), pick(o, wd & 3u), id);
+                const U4 o = philox10(lo, hi, sv.step, c3hop | (wd >> 4), k0, k1);
+                nx[s] += leap_draw(tau * ((du * (double)xu) * wq), coarse_byte(o, wd), lo, hi, sv.step, c3hop, k0,
+                                   k1, wd);
             }
         }
     }
@@ -844,7 +847,8 @@ __device__ __forceinline__ bool leap_subvolume(const KernelArgs& a, const NetVie
 // entries l, l+64, ... (coalesced loads, no divergence from uneven degrees),
 // finds the owning subvolume by binary search over the tile's row pointers
 // staged in LDS, and draws both hop directions of every species from the
-// edge's Philox block; then lane i draws the reactions of subvolume v0+i.
+// edge's coarse Philox block; then lane i draws the reactions of subvolume
+// v0+i (one coarse block per 16 reactions).
 // Balances of the tile's own subvolumes collect in LDS, those of other
 // tiles' subvolumes go straight to the global balances (gdelta); integer
 // atomics keep every sum exact and order-free.  phase_b_finish_kernel adds
@@ -852,11 +856,12 @@ __device__ __forceinline__ bool leap_subvolume(const KernelArgs& a, const NetVie
 //
 // Compacted draws.  Almost every draw of a step is 0 (hop means are ~1e-3
 // on the headline mesh), so phase_b_kernel's walk only applies the zero
-// shortcut: a draw it cannot settle as 0 goes to the wavefront's LDS queue
-// as its code and Philox word, and after the walk the whole wavefront draws
-// the queued ones, one per lane -- the float squeeze, then the double
-// search where the squeeze cannot decide -- so the search runs once per 64
-// non-zero draws instead of once per draw site.  A tile whose queue
+// shortcut to the coarse byte of each draw's word: a draw it cannot settle
+// as 0 goes to the wavefront's LDS queue as its code and coarse byte, and
+// after the walk the whole wavefront draws the queued ones, one per lane --
+// the fine bits, the float squeeze, then the double search where the
+// squeeze cannot decide -- so the fine block and the search run once per 64
+// open draws instead of once per draw site.  A tile whose queue
 // overflows, or that holds a draw of mean >= 10 (PTRS), applies nothing
 // from its queue and goes on the redo list: phase_b_redo_kernel walks it
 // again and draws every draw the zero shortcut leaves open (the same test
@@ -867,20 +872,22 @@ __device__ __forceinline__ bool leap_subvolume(const KernelArgs& a, const NetVie
 // ---------------------------------------------------------------------------
 constexpr uint32_t kQueue = 256;               // queued draws per wavefront
 constexpr uint32_t kQueueReact = 0x80000000u;  // reaction code: flag | lane << 8 | j
-constexpr uint32_t kQueueDefer = kQueue + 1u;  // count bump of a draw the queue does not take
+constexpr uint32_t kQueueDefer = kQueue + 1u;  // count bump that sends the tile to the redo pass
 constexpr float kPtrsSureF = 10.0001f;         // a float mean above it is a double mean >= 10
 constexpr uint32_t kMaxTileEdges = 1u << 27;   // hop codes hold the entry offset in 27 bits
 
-// A mean certainly >= 10 (PTRS) sends the tile to the redo pass; others
-// are queued with their word (the drain re-forms the mean and may still
-// defer).
-__device__ __forceinline__ void q_append(uint2* s_q, uint32_t* s_qn, bool ptrs, uint32_t code, uint32_t word) {
-    if (ptrs) {
-        atomicAdd(s_qn, kQueueDefer);
-        return;
+// Queue append by the whole wavefront (every lane calls it at the same
+// point, qn is wave-uniform): the lanes whose draw is open take consecutive
+// slots in lane order (a ballot and its prefix count, no LDS atomics).
+__device__ __forceinline__ void q_append(uint2* s_q, uint32_t& qn, bool open, uint32_t code, uint32_t word) {
+    const unsigned long long mq = __ballot(open);
+    if (mq == 0ull) return;
+    if (open) {
+        const uint32_t slot =
+            qn + __builtin_amdgcn_mbcnt_hi((uint32_t)(mq >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mq, 0u));
+        if (slot < kQueue) s_q[slot] = make_uint2(code, word);
     }
-    const uint32_t slot = atomicAdd(s_qn, 1u);
-    if (slot < kQueue) s_q[slot] = make_uint2(code, word);
+    qn += (uint32_t)__popcll(mq);
 }
 
 // Owner of entry e of the tile: last i with s_rp[i] <= e.
@@ -915,14 +922,18 @@ __device__ __forceinline__ void add_balance(const KernelArgs& a, const RealInfo&
 
 // The lane's draws: its forward entries (both directions of every species
 // of an undirected edge), then the reactions of its subvolume.  Walk: the
-// zero shortcut, draws it leaves open queued.  REDO: the open draws, with
-// the complete sampler, applied.
+// zero shortcut on the coarse bytes, draws it leaves open queued; returns
+// the queue count (wave-uniform; above kQueue when the tile goes to the
+// redo pass).  REDO: the open draws, with the complete sampler, applied.
+// Control flow stays wave-uniform around every queue append: the loops run
+// the same trip count on every lane, lanes without work carry flags.
 template <int MS, int MR, bool REDO>
-__device__ __forceinline__ void walk_fwd_draws(const KernelArgs& a, const NetView& n, const RealInfo& ri,
-                                               const uint32_t* X, uint32_t v0, const StepView& sv,
-                                               const uint32_t (*s_x)[kTile], unsigned long long (*s_del)[kTile],
-                                               const uint32_t* s_rp, const int32_t* s_c, uint2* s_q,
-                                               uint32_t* s_qn, unsigned long long& fire) {
+__device__ __forceinline__ uint32_t walk_fwd_draws(const KernelArgs& a, const NetView& n, const RealInfo& ri,
+                                                   const uint32_t* X, uint32_t v0, const StepView& sv,
+                                                   const uint32_t (*s_x)[kTile],
+                                                   unsigned long long (*s_del)[kTile], const uint32_t* s_rp,
+                                                   const int32_t* s_c, uint2* s_q, unsigned long long& fire) {
+    static_assert(MS <= 8, "one coarse block per edge; hop codes hold w = 2s + direction in 4 bits");
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t N = ri.N;
     const int nc = a.n_critical;
@@ -933,7 +944,9 @@ __device__ __forceinline__ void walk_fwd_draws(const KernelArgs& a, const NetVie
     const float tauf = (float)tau;
     const uint32_t k0 = a.seed_lo, k1 = a.seed_hi;
     const uint32_t c3hop = ctr_word3(sv.real, sv.attempt, kHop, 0u);
-    const uint32_t eb = s_rp[0], ee = s_rp[kTile];
+    const uint32_t eb = wave_uniform(s_rp[0]), ee = wave_uniform(s_rp[kTile]);
+    uint32_t qn = 0u;
+    bool defer = false;
     // software pipeline: an entry's neighbour counts are requested one
     // iteration ahead, its neighbour id and weight two iterations ahead
     // (counts of a neighbour inside the tile are read from global memory
@@ -947,8 +960,13 @@ __device__ __forceinline__ void walk_fwd_draws(const KernelArgs& a, const NetVie
     float w_cur = (e0 < ee && weighted) ? wp[e0] : 1.0f;
     float w_next = (e0 + (uint32_t)kTile < ee && weighted) ? wp[e0 + kTile] : 1.0f;
     uint32_t xn[MS];
+#pragma unroll
+    for (int s = 0; s < MS; ++s) xn[s] = 0u;
     if (e0 < ee) counts(u_cur, xn);
-    for (uint32_t e = e0; e < ee; e += (uint32_t)kTile) {
+    const uint32_t nit = (ee - eb + (uint32_t)kTile - 1u) / (uint32_t)kTile;
+    for (uint32_t it = 0; it < nit; ++it) {
+        const uint32_t e = e0 + it * (uint32_t)kTile;
+        const bool valid = e < ee;
         const uint32_t i = edge_owner(s_rp, e);
         const uint32_t vo = v0 + i;
         const uint32_t u = u_cur;  // u > vo: vo is the edge's lower end
@@ -967,61 +985,57 @@ __device__ __forceinline__ void walk_fwd_draws(const KernelArgs& a, const NetVie
         }
         const bool local = u - v0 < (uint32_t)kTile;
         const int co = n.C > 1 ? s_c[i] : 0;
-        const int cu = n.C > 1 ? (local ? s_c[u - v0] : a.comp[ri.comp_off + u]) : 0;
+        const int cu = (n.C > 1 && valid) ? (local ? s_c[u - v0] : a.comp[ri.comp_off + u]) : 0;
         const uint32_t ecode = (e - eb) << 4;
+        // the means of the edge's draws w = 2s + direction (0: vo -> u) and
+        // their coarse bounds
+        float lw[2 * MS], gw[2 * MS];
+        bool any = false;
 #pragma unroll
-        for (int s0 = 0; s0 < MS; s0 += 2) {
-            if (s0 >= n.S) break;
-            const bool two = s0 + 1 < MS && s0 + 1 < n.S;
-            const int s1 = s0 + 1 < MS ? s0 + 1 : s0;
-            const uint32_t xo0 = s_x[s0][i], xo1 = two ? s_x[s1][i] : 0u;
-            const double do0 = n.d[co * n.S + s0], do1 = two ? n.d[co * n.S + s1] : 0.0;
-            const uint32_t xu0 = xc[s0];
-            const uint32_t xu1 = two ? xc[s1] : 0u;
-            const double du0 = n.d[cu * n.S + s0], du1 = two ? n.d[cu * n.S + s1] : 0.0;
-            const uint32_t blk = (uint32_t)s0 >> 1;
-            const uint32_t cb = c3hop | blk;
-            // words: 2 (s & 1) + 0 is the hop vo -> u (vo < u), + 1 the hop u -> vo
-            const float lf0 = hop_mean_f(do0, xo0, wf, tauf, nc), lr0 = hop_mean_f(du0, xu0, wf, tauf, nc);
-            const float lf1 = two ? hop_mean_f(do1, xo1, wf, tauf, nc) : 0.0f;
-            const float lr1 = two ? hop_mean_f(du1, xu1, wf, tauf, nc) : 0.0f;
-            if (!(lf0 > 0.0f || lf1 > 0.0f || lr0 > 0.0f || lr1 > 0.0f)) continue;
-            const U4 o = philox10(vo, u, sv.step, cb, k0, k1);
+        for (int s = 0; s < MS; ++s) {
+            const bool on = valid && s < n.S;
+            const int sq = s < n.S ? s : 0;
+            lw[2 * s] = on ? hop_mean_f(n.d[co * n.S + sq], s_x[sq][i], wf, tauf, nc) : 0.0f;
+            lw[2 * s + 1] = on ? hop_mean_f(n.d[cu * n.S + sq], xc[s], wf, tauf, nc) : 0.0f;
+        }
+#pragma unroll
+        for (int w = 0; w < 2 * MS; ++w) {
+            gw[w] = coarse_bound(lw[w]);
+            any = any || lw[w] > 0.0f;
+            if constexpr (!REDO) defer = defer || lw[w] > kPtrsSureF;
+        }
+        if (__ballot(any) == 0ull) continue;
+        const U4 o = philox10(vo, u, sv.step, c3hop, k0, k1);  // the coarse block (MS <= 8)
+#pragma unroll
+        for (int w = 0; w < 2 * MS; ++w) {
+            const uint32_t hb = coarse_byte(o, (uint32_t)w);
+            const bool open = (float)hb > gw[w];
             if constexpr (REDO) {
-                long long f0 = 0, r0 = 0, f1 = 0, r1 = 0;  // forward / reverse hops
-                const double wq = (double)wf;
-                if (poisson_zero_f(lf0, o.x) < 0)
-                    f0 = poisson(hop_mean(do0, xo0, wq, tau, nc), o.x, DrawId{vo, u, sv.step, cb, k0, k1, 0u});
-                if (poisson_zero_f(lr0, o.y) < 0)
-                    r0 = poisson(hop_mean(du0, xu0, wq, tau, nc), o.y, DrawId{vo, u, sv.step, cb, k0, k1, 1u});
-                if (poisson_zero_f(lf1, o.z) < 0)
-                    f1 = poisson(hop_mean(do1, xo1, wq, tau, nc), o.z, DrawId{vo, u, sv.step, cb, k0, k1, 2u});
-                if (poisson_zero_f(lr1, o.w) < 0)
-                    r1 = poisson(hop_mean(du1, xu1, wq, tau, nc), o.w, DrawId{vo, u, sv.step, cb, k0, k1, 3u});
-                fire += (unsigned long long)(f0 + r0 + f1 + r1);
-                const long long b0 = r0 - f0, b1 = r1 - f1;
-                if (b0 != 0) {
-                    atomicAdd(&s_del[s0][i], (unsigned long long)b0);
-                    add_balance(a, ri, s_del, v0, s0, u, -b0);
-                }
-                if (b1 != 0) {
-                    atomicAdd(&s_del[s1][i], (unsigned long long)b1);
-                    add_balance(a, ri, s_del, v0, s1, u, -b1);
-                }
+                if (!open) continue;
+                const int s = w >> 1;
+                const bool fwd = (w & 1) == 0;
+                const uint32_t fb = c3hop | (kFineBlock + ((uint32_t)w >> 2));
+                const U4 f = philox10(vo, u, sv.step, fb, k0, k1);
+                const uint32_t word = draw_word(hb, f, (uint32_t)w);
+                if (poisson_zero_f(lw[w], word) >= 0) continue;
+                const double lam = fwd ? hop_mean(n.d[co * n.S + s], s_x[s][i], (double)wf, tau, nc)
+                                       : hop_mean(n.d[cu * n.S + s], xc[s], (double)wf, tau, nc);
+                const long long k = poisson(lam, word, DrawId{vo, u, sv.step, fb, k0, k1, (uint32_t)w & 3u});
+                if (k == 0) continue;
+                fire += (unsigned long long)k;
+                const long long bv = fwd ? -k : k;  // balance of vo
+                atomicAdd(&s_del[s][i], (unsigned long long)bv);
+                add_balance(a, ri, s_del, v0, s, u, -bv);
             } else {
-                const uint32_t wcode = ecode | (blk << 2);
-                if (poisson_zero_f(lf0, o.x) < 0) q_append(s_q, s_qn, lf0 > kPtrsSureF, wcode | 0u, o.x);
-                if (poisson_zero_f(lr0, o.y) < 0) q_append(s_q, s_qn, lr0 > kPtrsSureF, wcode | 1u, o.y);
-                if (poisson_zero_f(lf1, o.z) < 0) q_append(s_q, s_qn, lf1 > kPtrsSureF, wcode | 2u, o.z);
-                if (poisson_zero_f(lr1, o.w) < 0) q_append(s_q, s_qn, lr1 > kPtrsSureF, wcode | 3u, o.w);
+                q_append(s_q, qn, open, ecode | (uint32_t)w, hb);
             }
         }
     }
     const uint32_t v = v0 + lane;
-    if (v >= N) return;
+    const bool vok = v < N;
     // reactions of subvolume v, propensities and criticality as react_props
     // computes them (same operations, one reaction at a time)
-    const int c = n.C > 1 ? s_c[lane] : 0;
+    const int c = (n.C > 1 && vok) ? s_c[lane] : 0;
     uint32_t blk = 0xFFFFFFFFu;
     U4 o{0, 0, 0, 0};
     const uint32_t c3base = ctr_word3(sv.real, sv.attempt, kReact, 0u);
@@ -1035,35 +1049,38 @@ __device__ __forceinline__ void walk_fwd_draws(const KernelArgs& a, const NetVie
             aj = aj * binom_d(xs, rq);
             low = low || below_nc(xs, rq, nc);
         }
-        if ((n.order[j] > 0 && low) || !(aj > 0.0)) continue;
-        const uint32_t b = (uint32_t)j >> 2;
+        const bool live = vok && !(n.order[j] > 0 && low) && aj > 0.0;
+        const uint32_t b = (uint32_t)j >> 4;
         if (b != blk) {
             o = philox10(v, kSelfId, sv.step, c3base | b, k0, k1);
             blk = b;
         }
-        const uint32_t wd = (uint32_t)j & 3u;
+        const uint32_t hb = coarse_byte(o, (uint32_t)j);
         const double lam = tau * aj;
-        if (poisson_zero(lam, pick(o, wd)) >= 0) continue;
-        long long k;
+        const bool open = live && !coarse_zero(lam, hb);
         if constexpr (REDO) {
-            k = poisson(lam, pick(o, wd), DrawId{v, kSelfId, sv.step, c3base | b, k0, k1, wd});
+            if (!open) continue;
+            const long long k = leap_draw(lam, hb, v, kSelfId, sv.step, c3base, k0, k1, (uint32_t)j);
+            if (k == 0) continue;
+            fire += (unsigned long long)k;
+            for (int s = 0; s < n.S; ++s) {
+                const int nu = n.nu[s * n.R + j];
+                if (nu != 0) atomicAdd(&s_del[s][lane], (unsigned long long)(k * (long long)nu));
+            }
         } else {
-            q_append(s_q, s_qn, !(lam < kPoissonInvMax), kQueueReact | (lane << 8) | (uint32_t)j, pick(o, wd));
-            continue;
-        }
-        if (k == 0) continue;
-        fire += (unsigned long long)k;
-        for (int s = 0; s < n.S; ++s) {
-            const int nu = n.nu[s * n.R + j];
-            if (nu != 0) atomicAdd(&s_del[s][lane], (unsigned long long)(k * (long long)nu));
+            defer = defer || (open && !(lam < kPoissonInvMax));
+            q_append(s_q, qn, open, kQueueReact | (lane << 8) | (uint32_t)j, hb);
         }
     }
+    // a mean >= 10 (PTRS) anywhere: the tile goes to the redo pass
+    if (__ballot(defer) != 0ull) qn += kQueueDefer;
+    return qn;
 }
 
-// A queued hop, decoded: owner i, neighbour u, species s, word wd and the
-// mean in double.
+// A queued hop, decoded: owner i, neighbour u, draw w = 2s + direction,
+// species s and the mean in double.
 struct QueuedHop {
-    uint32_t i, u, blk, wd;
+    uint32_t i, u, w;
     int s;
     bool fwd;  // the hop v0 + i -> u
     double lam;
@@ -1076,13 +1093,12 @@ __device__ __forceinline__ QueuedHop queued_hop(const KernelArgs& a, const NetVi
                                                 const uint32_t (*s_x)[kTile], const uint32_t* s_rp,
                                                 const int32_t* s_c, uint32_t code) {
     QueuedHop h;
-    const uint32_t e = s_rp[0] + (code >> 4), w = code & 15u;
-    h.blk = w >> 2;
-    h.wd = w & 3u;
+    const uint32_t e = s_rp[0] + (code >> 4);
+    h.w = code & 15u;
     h.i = edge_owner(s_rp, e);
     h.u = a.fcol[ri.fe_off + e];
-    h.s = 2 * (int)h.blk + (int)(h.wd >> 1);
-    h.fwd = (h.wd & 1u) == 0u;
+    h.s = (int)(h.w >> 1);
+    h.fwd = (h.w & 1u) == 0u;
     const float wf = (ri.flags & 2u) ? a.fw[ri.fe_off + e] : 1.0f;
     const double wq = (double)wf;
     double d;
@@ -1101,40 +1117,57 @@ __device__ __forceinline__ QueuedHop queued_hop(const KernelArgs& a, const NetVi
     return h;
 }
 
-// One queued draw: the float squeeze on the mean and uniform the walk
-// formed, the double search where it cannot decide (the drain runs only
-// when every queued mean is below 10), applied to the balances.
+// One queued draw: its fine bits complete the word the walk queued the
+// coarse byte of (one Philox call, whichever the kind), then the float
+// squeeze on the mean rounded to float, the double search where it cannot
+// decide (the drain runs only when every queued mean is below 10), applied
+// to the balances.
 template <int MS>
 __device__ __forceinline__ void drain_fwd_draw(const KernelArgs& a, const NetView& n, const RealInfo& ri,
                                                const uint32_t* X, uint32_t v0, const StepView& sv,
                                                const uint32_t (*s_x)[kTile], unsigned long long (*s_del)[kTile],
                                                const uint32_t* s_rp, const int32_t* s_c, uint2 q,
                                                unsigned long long& fire) {
-    const uint32_t code = q.x, word = q.y;
-    if (code & kQueueReact) {
-        const uint32_t i = (code >> 8) & 63u, j = code & 255u;
-        double aj = n.k[(n.C > 1 ? s_c[i] : 0) * n.R + (int)j];
-        const int nr = n.nrs[j];
-        for (int r = 0; r < nr; ++r) aj = aj * binom_d(s_x[n.rs[j * 3 + r]][i], n.rr[j * 3 + r]);
-        const double lam = sv.tau * aj;
-        long long k = poisson_squeeze((float)lam, (float)uniform01(word));
-        if (k < 0) k = inversion_search(lam, word);
-        if (k == 0) return;
-        fire += (unsigned long long)k;
-        for (int s = 0; s < n.S; ++s) {
-            const int nu = n.nu[s * n.R + (int)j];
-            if (nu != 0) atomicAdd(&s_del[s][i], (unsigned long long)(k * (long long)nu));
-        }
-        return;
+    const uint32_t code = q.x, hb = q.y;
+    const bool react = (code & kQueueReact) != 0u;
+    uint32_t i, c1, w, c3;
+    float lf;
+    double lam;
+    QueuedHop h{};
+    if (react) {
+        i = (code >> 8) & 63u;
+        w = code & 255u;
+        double aj = n.k[(n.C > 1 ? s_c[i] : 0) * n.R + (int)w];
+        const int nr = n.nrs[w];
+        for (int r = 0; r < nr; ++r) aj = aj * binom_d(s_x[n.rs[w * 3 + r]][i], n.rr[w * 3 + r]);
+        lam = sv.tau * aj;
+        lf = (float)lam;
+        c1 = kSelfId;
+        c3 = ctr_word3(sv.real, sv.attempt, kReact, kFineBlock + (w >> 2));
+    } else {
+        h = queued_hop<MS>(a, n, ri, X, v0, sv, s_x, s_rp, s_c, code);
+        i = h.i;
+        w = h.w;
+        lam = h.lam;
+        lf = h.lf;
+        c1 = h.u;
+        c3 = ctr_word3(sv.real, sv.attempt, kHop, kFineBlock + (w >> 2));
     }
-    const QueuedHop h = queued_hop<MS>(a, n, ri, X, v0, sv, s_x, s_rp, s_c, code);
-    long long k = h.lf < kSqueezeMaxF ? poisson_squeeze(h.lf, __uint2float_rn(word) * 0x1p-32f) : -1;
-    if (k < 0) k = inversion_search(h.lam, word);
+    const uint32_t word = draw_word(hb, philox10(v0 + i, c1, sv.step, c3, a.seed_lo, a.seed_hi), w);
+    long long k = lf < kSqueezeMaxF ? poisson_squeeze(lf, __uint2float_rn(word) * 0x1p-32f) : -1;
+    if (k < 0) k = inversion_search(lam, word);
     if (k == 0) return;
     fire += (unsigned long long)k;
-    const long long b = h.fwd ? -k : k;  // balance of v0 + i
-    atomicAdd(&s_del[h.s][h.i], (unsigned long long)b);
-    add_balance(a, ri, s_del, v0, h.s, h.u, -b);
+    if (react) {
+        for (int s = 0; s < n.S; ++s) {
+            const int nu = n.nu[s * n.R + (int)w];
+            if (nu != 0) atomicAdd(&s_del[s][i], (unsigned long long)(k * (long long)nu));
+        }
+    } else {
+        const long long b = h.fwd ? -k : k;  // balance of v0 + i
+        atomicAdd(&s_del[h.s][i], (unsigned long long)b);
+        add_balance(a, ri, s_del, v0, h.s, h.u, -b);
+    }
 }
 
 // The draws of one tile into the global balances.  Returns false when the
@@ -1144,7 +1177,7 @@ template <int MS, int MR, bool REDO>
 __device__ __forceinline__ bool leap_tile_fwd(const KernelArgs& a, const NetView& n, const RealInfo& ri,
                                               const uint32_t* X, uint32_t v0, const StepView& sv,
                                               uint32_t (*s_x)[kTile], unsigned long long (*s_del)[kTile],
-                                              uint32_t* s_rp, int32_t* s_c, uint2* s_q, uint32_t* s_qn,
+                                              uint32_t* s_rp, int32_t* s_c, uint2* s_q,
                                               unsigned long long& fire) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t N = ri.N;
@@ -1157,16 +1190,13 @@ __device__ __forceinline__ bool leap_tile_fwd(const KernelArgs& a, const NetView
     }
     s_c[lane] = (n.C > 1 && active) ? a.comp[ri.comp_off + v] : 0;
     s_rp[lane] = a.frp[ri.rp_off + min(v, N)];
-    if (lane == 0) {
-        s_rp[kTile] = a.frp[ri.rp_off + min(v0 + (uint32_t)kTile, N)];
-        if (!REDO) *s_qn = 0u;
-    }
+    if (lane == 0) s_rp[kTile] = a.frp[ri.rp_off + min(v0 + (uint32_t)kTile, N)];
     lds_wave_sync();
-    walk_fwd_draws<MS, MR, REDO>(a, n, ri, X, v0, sv, s_x, s_del, s_rp, s_c, s_q, s_qn, fire);
+    const uint32_t qn = walk_fwd_draws<MS, MR, REDO>(a, n, ri, X, v0, sv, s_x, s_del, s_rp, s_c, s_q, fire);
     lds_wave_sync();
     bool complete = true;
     if constexpr (!REDO) {
-        const uint32_t cnt = wave_uniform(*s_qn);
+        const uint32_t cnt = wave_uniform(qn);
         complete = cnt <= kQueue;
         if (complete) {
             // a queued hop whose double mean is >= 10 needs PTRS: nothing
@@ -1329,7 +1359,7 @@ __device__ __forceinline__ StepView step_view(const RealInfo& ri, cptr<RealCtl> 
 template <int MS, int MR, bool EP, bool REDO>
 __device__ __forceinline__ void phase_b_tile(const KernelArgs& a, const NetView& n, uint32_t t,
                                              uint32_t (*s_x)[kTile], unsigned long long (*s_del)[kTile],
-                                             uint32_t* s_rp, int32_t* s_c, uint2* s_q, uint32_t* s_qn) {
+                                             uint32_t* s_rp, int32_t* s_c, uint2* s_q) {
     const uint32_t lane = threadIdx.x & 63u;
     const TileEnt te = ld_ro(a.tiles + t);
     const uint32_t r = wave_uniform(te.r);
@@ -1343,7 +1373,7 @@ __device__ __forceinline__ void phase_b_tile(const KernelArgs& a, const NetView&
     unsigned long long fire = 0;
     uint32_t bad = 0, viol = 0;
     if constexpr (EP) {
-        const bool complete = leap_tile_fwd<MS, MR, REDO>(a, n, ri, X, te.v0, sv, s_x, s_del, s_rp, s_c, s_q, s_qn, fire);
+        const bool complete = leap_tile_fwd<MS, MR, REDO>(a, n, ri, X, te.v0, sv, s_x, s_del, s_rp, s_c, s_q, fire);
         reduce_partB(fire, bad, viol);
         if (lane == 0) {
             if (!complete) {
@@ -1376,11 +1406,10 @@ void phase_b_kernel(KernelArgs a) {
     __shared__ uint32_t s_rp[4][kTile + 1];
     __shared__ int32_t s_c[4][kTile];
     __shared__ uint2 s_q[4][EP ? kQueue : 1];
-    __shared__ uint32_t s_qn[4];
     for (uint32_t it = 0;; ++it) {
         const uint32_t t = tile_at(it, a.ntiles, a.waves_per_block);
         if (t == 0xFFFFFFFFu) break;
-        phase_b_tile<MS, MR, EP, false>(a, n, t, s_x[wv], s_del[wv], s_rp[wv], s_c[wv], s_q[wv], &s_qn[wv]);
+        phase_b_tile<MS, MR, EP, false>(a, n, t, s_x[wv], s_del[wv], s_rp[wv], s_c[wv], s_q[wv]);
     }
 }
 
@@ -1399,7 +1428,7 @@ __global__ __launch_bounds__(256) void phase_b_redo_kernel(KernelArgs a) {
     if (blockIdx.x == 0 && threadIdx.x == 0 && cnt != 0u) *a.redo_total += cnt;
     for (uint32_t q = blockIdx.x * wpb + wv; q < cnt; q += gridDim.x * wpb)
         phase_b_tile<MS, MR, true, true>(a, n, wave_uniform(a.redo_tiles[q]), s_x[wv], s_del[wv], s_rp[wv], s_c[wv],
-                                         nullptr, nullptr);
+                                         nullptr);
 }
 
 // The end of the edge-parallel leap pass: every tile adds the global

@@ -60,6 +60,45 @@ __device__ __forceinline__ uint32_t pick(const U4& o, uint32_t i) {
     return i == 0 ? o.x : (i == 1 ? o.y : (i == 2 ? o.z : o.w));
 }
 
+// Uniforms of the leap draws.  Draw w of a counter (hop w = 2s + direction
+// of edge {c0 < c1}, direction 0 for the hop c0 -> c1; reaction w = j of
+// subvolume c0 with c1 = kSelfId) takes the 32-bit word
+//     (coarse byte << 24) | (fine bits & 0xFFFFFF):
+// the coarse byte is byte w & 15 of the Philox block numbered w >> 4 (16
+// draws per block: bytes 0..3 of word 0, then of words 1, 2, 3), the fine
+// bits word w & 3 of the block numbered kFineBlock + (w >> 2).  A draw
+// whose coarse byte already proves it 0 (coarse_zero*) never needs its fine
+// block, so the common no-event draw costs a sixteenth of a Philox call.
+constexpr uint32_t kFineBlock = 64u;
+
+__device__ __forceinline__ uint32_t coarse_byte(const U4& o, uint32_t w) {
+    return (pick(o, (w >> 2) & 3u) >> (8u * (w & 3u))) & 0xFFu;
+}
+
+__device__ __forceinline__ uint32_t draw_word(uint32_t hb, const U4& fine, uint32_t w) {
+    return (hb << 24) | (pick(fine, w & 3u) & 0xFFFFFFu);
+}
+
+// Every word with top byte hb is at most (hb + 1) 2^24 - 1, so its uniform
+// is below (hb + 1) 2^-8 (exact in double): when that bound passes the
+// zero shortcut of poisson() (double mean), the draw is 0 whatever its
+// fine bits.
+__device__ __forceinline__ bool coarse_zero(double lam, uint32_t hb) {
+    return !(lam > 0.0) || (double)(hb + 1u) * 0x1p-8 <= (1.0 - lam) - 0x1p-50;
+}
+
+// The same decision for a mean lf formed in single precision (within
+// 2^-21 relative of the double mean lam, as hop_mean_f forms it), as one
+// compare per draw: the draw is 0 when (float)hb <= coarse_bound(lf), with
+// g = (255 - 2^-12) - lf (256 + 2^-10) in float for lf > 0, 256 otherwise.
+// Then 255 - hb > lf 256 (1 + 2^-18)(1 - 2^-24) (g carries at most 2^-17
+// of rounding, less than the 2^-12 taken off), so (hb + 1) 2^-8 < 1 - lam
+// <= e^-lam: every word with that top byte gives 0.  A mean above 1 is
+// never settled here.
+__device__ __forceinline__ float coarse_bound(float lf) {
+    return lf > 0.0f ? 0x1.fdffe0p+7f - lf * 0x1.00004p+8f : 256.0f;
+}
+
 __device__ __forceinline__ double uniform01(uint32_t w) { return ((double)w + 0.5) * 0x1p-32; }
 
 // exp(x): Cody-Waite reduction r = x - n ln2 (two fma steps), degree-13
@@ -275,6 +314,18 @@ __device__ __forceinline__ long long poisson(double lam, uint32_t u0, const Draw
         if (k >= 0) return k;
     }
     return poisson_slow(lam, u0, id);
+}
+
+// Leap draw w of counter (c0, c1, step, c3base | block) with mean lam and
+// coarse byte hb, complete sampler: 0 when the coarse byte decides it, else
+// the fine block's bits complete the word (PTRS draws its extra uniforms
+// from the fine block's stream).
+__device__ __forceinline__ long long leap_draw(double lam, uint32_t hb, uint32_t c0, uint32_t c1, uint32_t step,
+                                               uint32_t c3base, uint32_t k0, uint32_t k1, uint32_t w) {
+    if (coarse_zero(lam, hb)) return 0;
+    const uint32_t fb = c3base | (kFineBlock + (w >> 2));
+    const U4 f = philox10(c0, c1, step, fb, k0, k1);
+    return poisson(lam, draw_word(hb, f, w), DrawId{c0, c1, step, fb, k0, k1, w & 3u});
 }
 
 // Largest single-precision mean the float squeeze takes: a float mean

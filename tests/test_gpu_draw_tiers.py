@@ -1,9 +1,9 @@
 """The compacted sampler of the edge-parallel leap pass.
 
-phase_b_kernel settles draws with the zero shortcut only, queues the rest
-per wavefront (at most 256, with their Philox words) and draws them after
-the walk, one per lane (float squeeze, double search where it cannot
-decide); a tile with a mean >= 10 draw or a full queue applies nothing from
+phase_b_kernel settles draws with the zero shortcut on the top byte of
+their words only, queues the rest per wavefront (at most 256, with their
+top bytes) and draws them after the walk, one per lane (fine bits, float
+squeeze, double search where it cannot decide); a tile with a mean >= 10 draw or a full queue applies nothing from
 its queue and phase_b_redo_kernel draws its open draws with the complete
 sampler.  Each undirected edge belongs to the tile of its lower end, which
 draws both of its hop directions.  These cases drive each branch on purpose
@@ -59,27 +59,33 @@ def _tune_eps(net, topo, x0, lam_of_tau, lo_t, hi_t):
     raise AssertionError("no eps found")
 
 
-def _hop_zero_f(tau, d, x, word):
-    """The walk's zero shortcut for a hop, in single precision like the
-    kernel: mean tau_f ((d_f x_f) w) with w = 1, u_f = word rounded to float
-    times 2^-32, settled iff u_f <= (1 - l) - 2^-19."""
+def _coarse_byte(o, w):
+    """Byte w & 15 of a Philox block: bytes 0..3 of word 0, then words 1..3."""
+    return (o[(w >> 2) & 3] >> (8 * (w & 3))) & 0xFF
+
+
+def _hop_coarse_zero_f(tau, d, x, hb):
+    """The walk's zero shortcut for a hop, on the top byte of its word and in
+    single precision like the kernel: mean l = tau_f ((d_f x_f) w) with
+    w = 1, settled iff hb <= (255 - 2^-12) - l (256 + 2^-10)."""
     lf = F32(tau) * (F32(F32(d) * F32(x)) * F32(1.0))
     if not lf > 0:
         return True
-    u = F32(F32(np.uint32(word)) * F32(2.0 ** -32))
-    return bool(u <= F32(F32(F32(1.0) - lf) - F32(2.0 ** -19)))
+    g = F32(F32(255.0 - 2.0 ** -12) - F32(lf * F32(256.0 + 2.0 ** -10)))
+    return bool(F32(hb) <= g)
 
 
-def _react_zero(lam, word):
+def _react_coarse_zero(lam, hb):
     if not lam > 0.0:
         return True
-    return (word + 0.5) * 2.0 ** -32 <= (1.0 - lam) - 2.0 ** -50
+    return (hb + 1) * 2.0 ** -8 <= (1.0 - lam) - 2.0 ** -50
 
 
 def _predict_step0(net, topo, x0, eps, seed=1, nc=10):
     """Per tile of the first leap pass: number of draws the zero shortcut
     leaves open (queued) and whether a mean >= 10 draw occurs.  Unweighted
-    symmetric topology, one compartment, realization 0, step 0, attempt 0."""
+    symmetric topology, one compartment, at most 8 species, realization 0,
+    step 0, attempt 0."""
     S, R, N = net["S"], net["R"], topo["N"]
     x = x0.reshape(S, N).astype(np.int64)
     d = np.asarray(net["d"], dtype=float)
@@ -100,18 +106,14 @@ def _predict_step0(net, topo, x0, eps, seed=1, nc=10):
             u = col[q]
             if u < v:
                 continue  # each undirected edge is drawn by the tile of its lower end
-            for blk in range((S + 1) // 2):
-                o = ffi.philox((v, u, 0, (P_HOP << 7) | blk), key)
-                for s in (2 * blk, 2 * blk + 1):
-                    if s >= S:
-                        continue
-                    for src, dirbit in ((v, 0), (u, 1)):
-                        xs = x[s, src]
-                        if d[s] > 0 and xs >= nc and xs > 0:
-                            word = o[2 * (s & 1) + dirbit]
-                            if not _hop_zero_f(tau, d[s], xs, word):
-                                und[t] += 1
-                                big[t] |= tau * ((d[s] * float(xs)) * 1.0) >= 10.0
+            o = ffi.philox((v, u, 0, P_HOP << 7), key)  # the edge's coarse block
+            for s in range(S):
+                for src, dirbit in ((v, 0), (u, 1)):
+                    xs = x[s, src]
+                    if d[s] > 0 and xs >= nc and xs > 0:
+                        if not _hop_coarse_zero_f(tau, d[s], xs, _coarse_byte(o, 2 * s + dirbit)):
+                            und[t] += 1
+                            big[t] |= tau * ((d[s] * float(xs)) * 1.0) >= 10.0
         o, oblk = None, -1
         for j in range(R):
             aj = k[j]
@@ -123,9 +125,9 @@ def _predict_step0(net, topo, x0, eps, seed=1, nc=10):
                     L = x[s, v] // r if L < 0 else min(L, x[s, v] // r)
             if (order[j] > 0 and L < nc) or not aj > 0.0:
                 continue
-            if j >> 2 != oblk:
-                o, oblk = ffi.philox((v, 0xFFFFFFFF, 0, (P_REACT << 7) | (j >> 2)), key), j >> 2
-            if not _react_zero(tau * aj, o[j & 3]):
+            if j >> 4 != oblk:
+                o, oblk = ffi.philox((v, 0xFFFFFFFF, 0, (P_REACT << 7) | (j >> 4)), key), j >> 4
+            if not _react_coarse_zero(tau * aj, _coarse_byte(o, j)):
                 und[t] += 1
                 big[t] |= tau * aj >= 10.0
     return und, big

@@ -156,3 +156,81 @@ def test_leap_back_keeps_populations_valid():
     st = ffi.run(m, x, 0.9, 2, 1, np.inf, step_limit=400)
     assert st.leapbacks >= 1
     assert x.max() < 10**6  # a negative count would have wrapped around
+
+
+def _coarse_fine_word(c0, c1, purpose, w, key, step=0):
+    """The leap draw's 32-bit word: top byte = byte w & 15 of Philox block
+    w >> 4 (bytes 0..3 of word 0, then words 1..3), low 24 bits = word w & 3
+    of block 64 + (w >> 2); realization 0, attempt 0."""
+    o = ffi.philox([c0, c1, step, (purpose << 7) | (w >> 4)], key)
+    hb = (o[(w >> 2) & 3] >> (8 * (w & 3))) & 0xFF
+    f = ffi.philox([c0, c1, step, (purpose << 7) | (64 + (w >> 2))], key)
+    return (hb << 24) | (f[w & 3] & 0xFFFFFF)
+
+
+def _inversion(lam, word):
+    """Poisson(lam) by sequential-search inversion of u = (word + 1/2) 2^-32,
+    the CDF summed in exact rationals from a long-double e^-lam."""
+    u = Fraction(2 * word + 1, 2 ** 33)
+    p = Fraction(str(np.exp(np.longdouble(-lam))))
+    F, k, lamf = p, 0, Fraction(lam)
+    while u > F:
+        k += 1
+        p = p * lamf / k
+        F += p
+    return k
+
+
+@pytest.mark.parametrize("seed", [1, 7, 23])
+def test_leap_hop_draws_by_hand(seed):
+    """Two subvolumes, one diffusing species, one step: the hops 0 -> 1 and
+    1 -> 0 are the inversion of the words built by hand from Philox (purpose
+    4, counter (0, 1, step 0), draws w = 0 and 1), with the means tau d x w
+    and tau from the oracle's own tau' (checked against its closed form)."""
+    net = W.network(["X"], [], np.zeros(0), [0.5])
+    topo = {"N": 2, "row_ptr": np.array([0, 1, 2], np.uint32), "col": np.array([1, 0], np.uint32),
+            "w": np.array([1.5, 1.5], np.float32)}
+    m = ffi.Model(net, topo)
+    x0 = np.array([4000, 1000], np.uint32)
+    eps = 0.001
+    tp, a0c, _ = ffi.tau_all(m, x0, eps, 10, seed)
+    # mu_0 = 0.75 (1000 - 4000), s2_0 = 0.75 * 5000, bound eps x_0 = 4
+    assert tp[0] == pytest.approx(min(4 / 2250, 4 ** 2 / 3750), rel=1e-15)
+    tau = float(min(tp))
+    assert a0c.max() == 0.0
+    key = [seed & 0xFFFFFFFF, seed >> 32]
+    lam01, lam10 = tau * ((0.5 * 4000.0) * 1.5), tau * ((0.5 * 1000.0) * 1.5)
+    k01 = _inversion(lam01, _coarse_fine_word(0, 1, 4, 0, key))
+    k10 = _inversion(lam10, _coarse_fine_word(0, 1, 4, 1, key))
+    assert 0 < lam10 < lam01 < 10.0
+    x = x0.copy()
+    st = ffi.run(m, x, eps, 10, seed, np.inf, step_limit=1)
+    assert st.steps == 1 and st.leapbacks == 0
+    assert list(x) == [4000 - k01 + k10, 1000 + k01 - k10], (k01, k10)
+    assert st.firings == k01 + k10
+
+
+@pytest.mark.parametrize("seed", [2, 5])
+def test_leap_reaction_draws_by_hand(seed):
+    """One subvolume, 18 first-order channels X -> Y_j (so the draws of
+    reactions 16 and 17 come from the second coarse block): each Y_j after one
+    step is the inversion of its word built by hand (purpose 3, counter
+    (0, 0xFFFFFFFF, step 0), draw w = j), and X lost their sum."""
+    R = 18
+    species = ["X"] + [f"Y{j}" for j in range(R)]
+    reactions = [({"X": 1}, {f"Y{j}": 1}) for j in range(R)]
+    rates = [0.01 * (j + 1) for j in range(R)]
+    net = W.network(species, reactions, rates, [0.0] * (R + 1))
+    m = ffi.Model(net, SINGLE)
+    x0 = np.zeros(R + 1, np.uint32)
+    x0[0] = 500
+    eps = 0.05
+    tp, _, _ = ffi.tau_all(m, x0, eps, 10, seed)
+    tau = float(tp[0])
+    key = [seed & 0xFFFFFFFF, seed >> 32]
+    x = x0.copy()
+    st = ffi.run(m, x, eps, 10, seed, np.inf, step_limit=1)
+    assert st.steps == 1
+    want = [_inversion(tau * (rates[j] * 500.0), _coarse_fine_word(0, 0xFFFFFFFF, 3, j, key)) for j in range(R)]
+    assert list(x[1:]) == want
+    assert x[0] == 500 - sum(want) and sum(want[16:]) > 0
