@@ -81,6 +81,7 @@ struct plis_ctx {
     double* d_wsum = nullptr;  // per subvolume, row_ptr layout
     uint32_t *d_frp = nullptr, *d_fcol = nullptr;  // forward CSR (symmetric realizations)
     float* d_fw = nullptr;
+    uint8_t* d_fown = nullptr;  // forward entry -> its row's index in the row's 64-subvolume tile
     unsigned long long* d_gdelta = nullptr;        // leap balances, counts layout
     float *d_w = nullptr, *d_iw = nullptr;
     int32_t* d_comp = nullptr;
@@ -328,6 +329,7 @@ static KernelArgs make_args(plis_ctx* c) {
     a.frp = c->d_frp;
     a.fcol = c->d_fcol;
     a.fw = c->d_fw;
+    a.fown = c->d_fown;
     a.gdelta = c->d_gdelta;
     a.redo_total = reinterpret_cast<unsigned long long*>(c->d_cnt + 2);
     a.epsilon = c->eps;
@@ -569,6 +571,7 @@ extern "C" int plis_create(const plis_network* net, int32_t n_real, const plis_t
     // host-side layout
     std::vector<uint32_t> hx, hrp, hcol, hirp, hicol, hfrp, hfcol;
     std::vector<float> hfw;
+    std::vector<uint8_t> hfown;
     std::vector<float> hw, hiw;
     std::vector<double> hwsum;
     std::vector<int32_t> hcomp;
@@ -611,6 +614,7 @@ extern "C" int plis_create(const plis_network* net, int32_t n_real, const plis_t
                 if (g.col[q] > v) {
                     hfcol.push_back(g.col[q]);
                     hfw.push_back(g.w[q]);
+                    hfown.push_back((uint8_t)(v % kTile));
                 }
         }
         hfrp.push_back((uint32_t)(hfcol.size() - ri.fe_off));
@@ -711,6 +715,7 @@ extern "C" int plis_create(const plis_network* net, int32_t n_real, const plis_t
     const bool ep = !any_asym && !huge_tile;
     if (ep && ((rc = upload(c, &c->d_frp, hfrp)) || (rc = upload(c, &c->d_fcol, hfcol)) ||
                       (rc = upload(c, &c->d_fw, any_weighted ? hfw : std::vector<float>())) ||
+                      (rc = upload(c, &c->d_fown, hfown)) ||
                       (rc = upload(c, &c->d_gdelta, std::vector<unsigned long long>(hx.size(), 0ull))))) {
         g_create_err = c->err;
         plis_destroy(c);

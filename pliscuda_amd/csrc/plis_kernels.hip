@@ -845,8 +845,8 @@ __device__ __forceinline__ bool leap_subvolume(const KernelArgs& a, const NetVie
 // Each undirected edge {v, u}, v < u, is drawn once, by the tile of its
 // lower end: the tile's forward-CSR entries are contiguous, lane l takes
 // entries l, l+64, ... (coalesced loads, no divergence from uneven degrees),
-// finds the owning subvolume by binary search over the tile's row pointers
-// staged in LDS, and draws both hop directions of every species from the
+// reads the owning subvolume's index in the tile from the per-entry owner
+// bytes (fown), and draws both hop directions of every species from the
 // edge's coarse Philox block; then lane i draws the reactions of subvolume
 // v0+i (one coarse block per 16 reactions).
 // Balances of the tile's own subvolumes collect in LDS, those of other
@@ -890,18 +890,6 @@ __device__ __forceinline__ void q_append(uint2* s_q, uint32_t& qn, bool open, ui
     qn += (uint32_t)__popcll(mq);
 }
 
-// Owner of entry e of the tile: last i with s_rp[i] <= e.
-__device__ __forceinline__ uint32_t edge_owner(const uint32_t* s_rp, uint32_t e) {
-    uint32_t lo_i = 0, hi_i = kTile - 1;
-#pragma unroll
-    for (int it = 0; it < 6; ++it) {
-        const uint32_t mid = (lo_i + hi_i + 1u) >> 1;
-        if (s_rp[mid] <= e) lo_i = mid;
-        else hi_i = mid - 1u;
-    }
-    return lo_i;
-}
-
 __device__ __forceinline__ double hop_mean(double d, uint32_t x, double wq, double tau, int nc) {
     return hop_active(d, x, nc) ? tau * ((d * (double)x) * wq) : 0.0;
 }
@@ -931,7 +919,7 @@ template <int MS, int MR, bool REDO>
 __device__ __forceinline__ uint32_t walk_fwd_draws(const KernelArgs& a, const NetView& n, const RealInfo& ri,
                                                    const uint32_t* X, uint32_t v0, const StepView& sv,
                                                    const uint32_t (*s_x)[kTile],
-                                                   unsigned long long (*s_del)[kTile], const uint32_t* s_rp,
+                                                   unsigned long long (*s_del)[kTile], uint32_t eb, uint32_t ee,
                                                    const int32_t* s_c, uint2* s_q, unsigned long long& fire) {
     static_assert(MS <= 8, "one coarse block per edge; hop codes hold w = 2s + direction in 4 bits");
     const uint32_t lane = threadIdx.x & 63u;
@@ -944,9 +932,19 @@ __device__ __forceinline__ uint32_t walk_fwd_draws(const KernelArgs& a, const Ne
     const float tauf = (float)tau;
     const uint32_t k0 = a.seed_lo, k1 = a.seed_hi;
     const uint32_t c3hop = ctr_word3(sv.real, sv.attempt, kHop, 0u);
-    const uint32_t eb = wave_uniform(s_rp[0]), ee = wave_uniform(s_rp[kTile]);
     uint32_t qn = 0u;
     bool defer = false;
+    // one compartment: the diffusion constants as float and the species
+    // that hop at all (hop_mean_f's operands and test, loaded once)
+    const uint32_t nc1 = nc > 1 ? (uint32_t)nc : 1u;
+    float df[MS];
+    bool dpos[MS];
+#pragma unroll
+    for (int s = 0; s < MS; ++s) {
+        const double d = n.d[s < n.S ? s : 0];
+        df[s] = (float)d;
+        dpos[s] = s < n.S && d > 0.0;
+    }
     // software pipeline: an entry's neighbour counts are requested one
     // iteration ahead, its neighbour id and weight two iterations ahead
     // (counts of a neighbour inside the tile are read from global memory
@@ -955,10 +953,12 @@ __device__ __forceinline__ uint32_t walk_fwd_draws(const KernelArgs& a, const Ne
 #pragma unroll
         for (int s = 0; s < MS; ++s) xo[s] = X[ri.x_off + (uint64_t)(s < n.S ? s : n.S - 1) * N + u];
     };
+    const uint8_t* ownp = a.fown + ri.fe_off;
     const uint32_t e0 = eb + lane;
     uint32_t u_cur = e0 < ee ? colp[e0] : v0, u_next = e0 + (uint32_t)kTile < ee ? colp[e0 + kTile] : v0;
     float w_cur = (e0 < ee && weighted) ? wp[e0] : 1.0f;
     float w_next = (e0 + (uint32_t)kTile < ee && weighted) ? wp[e0 + kTile] : 1.0f;
+    uint32_t i_cur = e0 < ee ? ownp[e0] : 0u, i_next = e0 + (uint32_t)kTile < ee ? ownp[e0 + kTile] : 0u;
     uint32_t xn[MS];
 #pragma unroll
     for (int s = 0; s < MS; ++s) xn[s] = 0u;
@@ -967,7 +967,7 @@ __device__ __forceinline__ uint32_t walk_fwd_draws(const KernelArgs& a, const Ne
     for (uint32_t it = 0; it < nit; ++it) {
         const uint32_t e = e0 + it * (uint32_t)kTile;
         const bool valid = e < ee;
-        const uint32_t i = edge_owner(s_rp, e);
+        const uint32_t i = i_cur;  // the edge's row in the tile
         const uint32_t vo = v0 + i;
         const uint32_t u = u_cur;  // u > vo: vo is the edge's lower end
         const float wf = w_cur;
@@ -976,11 +976,13 @@ __device__ __forceinline__ uint32_t walk_fwd_draws(const KernelArgs& a, const Ne
         for (int s = 0; s < MS; ++s) xc[s] = xn[s];
         u_cur = u_next;
         w_cur = w_next;
+        i_cur = i_next;
         if (e + (uint32_t)kTile < ee) {
             counts(u_cur, xn);
             if (e + 2u * (uint32_t)kTile < ee) {
                 u_next = colp[e + 2u * kTile];
                 w_next = weighted ? wp[e + 2u * kTile] : 1.0f;
+                i_next = ownp[e + 2u * kTile];
             }
         }
         const bool local = u - v0 < (uint32_t)kTile;
@@ -993,10 +995,17 @@ __device__ __forceinline__ uint32_t walk_fwd_draws(const KernelArgs& a, const Ne
         bool any = false;
 #pragma unroll
         for (int s = 0; s < MS; ++s) {
-            const bool on = valid && s < n.S;
             const int sq = s < n.S ? s : 0;
-            lw[2 * s] = on ? hop_mean_f(n.d[co * n.S + sq], s_x[sq][i], wf, tauf, nc) : 0.0f;
-            lw[2 * s + 1] = on ? hop_mean_f(n.d[cu * n.S + sq], xc[s], wf, tauf, nc) : 0.0f;
+            if (n.C > 1) {
+                const bool on = valid && s < n.S;
+                lw[2 * s] = on ? hop_mean_f(n.d[co * n.S + sq], s_x[sq][i], wf, tauf, nc) : 0.0f;
+                lw[2 * s + 1] = on ? hop_mean_f(n.d[cu * n.S + sq], xc[s], wf, tauf, nc) : 0.0f;
+            } else {
+                const bool on = valid && dpos[s];
+                const uint32_t xo = s_x[sq][i], xu = xc[s];
+                lw[2 * s] = (on && xo >= nc1) ? tauf * ((df[s] * (float)xo) * wf) : 0.0f;
+                lw[2 * s + 1] = (on && xu >= nc1) ? tauf * ((df[s] * (float)xu) * wf) : 0.0f;
+            }
         }
 #pragma unroll
         for (int w = 0; w < 2 * MS; ++w) {
@@ -1090,12 +1099,12 @@ struct QueuedHop {
 template <int MS>
 __device__ __forceinline__ QueuedHop queued_hop(const KernelArgs& a, const NetView& n, const RealInfo& ri,
                                                 const uint32_t* X, uint32_t v0, const StepView& sv,
-                                                const uint32_t (*s_x)[kTile], const uint32_t* s_rp,
+                                                const uint32_t (*s_x)[kTile], uint32_t eb,
                                                 const int32_t* s_c, uint32_t code) {
     QueuedHop h;
-    const uint32_t e = s_rp[0] + (code >> 4);
+    const uint32_t e = eb + (code >> 4);
     h.w = code & 15u;
-    h.i = edge_owner(s_rp, e);
+    h.i = a.fown[ri.fe_off + e];
     h.u = a.fcol[ri.fe_off + e];
     h.s = (int)(h.w >> 1);
     h.fwd = (h.w & 1u) == 0u;
@@ -1126,7 +1135,7 @@ template <int MS>
 __device__ __forceinline__ void drain_fwd_draw(const KernelArgs& a, const NetView& n, const RealInfo& ri,
                                                const uint32_t* X, uint32_t v0, const StepView& sv,
                                                const uint32_t (*s_x)[kTile], unsigned long long (*s_del)[kTile],
-                                               const uint32_t* s_rp, const int32_t* s_c, uint2 q,
+                                               uint32_t eb, const int32_t* s_c, uint2 q,
                                                unsigned long long& fire) {
     const uint32_t code = q.x, hb = q.y;
     const bool react = (code & kQueueReact) != 0u;
@@ -1145,7 +1154,7 @@ __device__ __forceinline__ void drain_fwd_draw(const KernelArgs& a, const NetVie
         c1 = kSelfId;
         c3 = ctr_word3(sv.real, sv.attempt, kReact, kFineBlock + (w >> 2));
     } else {
-        h = queued_hop<MS>(a, n, ri, X, v0, sv, s_x, s_rp, s_c, code);
+        h = queued_hop<MS>(a, n, ri, X, v0, sv, s_x, eb, s_c, code);
         i = h.i;
         w = h.w;
         lam = h.lam;
@@ -1177,7 +1186,7 @@ template <int MS, int MR, bool REDO>
 __device__ __forceinline__ bool leap_tile_fwd(const KernelArgs& a, const NetView& n, const RealInfo& ri,
                                               const uint32_t* X, uint32_t v0, const StepView& sv,
                                               uint32_t (*s_x)[kTile], unsigned long long (*s_del)[kTile],
-                                              uint32_t* s_rp, int32_t* s_c, uint2* s_q,
+                                              int32_t* s_c, uint2* s_q,
                                               unsigned long long& fire) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t N = ri.N;
@@ -1189,10 +1198,10 @@ __device__ __forceinline__ bool leap_tile_fwd(const KernelArgs& a, const NetView
         s_del[s][lane] = 0ull;
     }
     s_c[lane] = (n.C > 1 && active) ? a.comp[ri.comp_off + v] : 0;
-    s_rp[lane] = a.frp[ri.rp_off + min(v, N)];
-    if (lane == 0) s_rp[kTile] = a.frp[ri.rp_off + min(v0 + (uint32_t)kTile, N)];
+    const uint32_t eb = wave_uniform(a.frp[ri.rp_off + v0]);
+    const uint32_t ee = wave_uniform(a.frp[ri.rp_off + min(v0 + (uint32_t)kTile, N)]);
     lds_wave_sync();
-    const uint32_t qn = walk_fwd_draws<MS, MR, REDO>(a, n, ri, X, v0, sv, s_x, s_del, s_rp, s_c, s_q, fire);
+    const uint32_t qn = walk_fwd_draws<MS, MR, REDO>(a, n, ri, X, v0, sv, s_x, s_del, eb, ee, s_c, s_q, fire);
     lds_wave_sync();
     bool complete = true;
     if constexpr (!REDO) {
@@ -1205,13 +1214,13 @@ __device__ __forceinline__ bool leap_tile_fwd(const KernelArgs& a, const NetView
             for (uint32_t q = lane; q < cnt; q += (uint32_t)kTile) {
                 const uint32_t code = s_q[q].x;
                 if (!(code & kQueueReact))
-                    big = big || !(queued_hop<MS>(a, n, ri, X, v0, sv, s_x, s_rp, s_c, code).lam < kPoissonInvMax);
+                    big = big || !(queued_hop<MS>(a, n, ri, X, v0, sv, s_x, eb, s_c, code).lam < kPoissonInvMax);
             }
             complete = __ballot(big) == 0ull;
         }
         if (complete) {
             for (uint32_t q = lane; q < cnt; q += (uint32_t)kTile)
-                drain_fwd_draw<MS>(a, n, ri, X, v0, sv, s_x, s_del, s_rp, s_c, s_q[q], fire);
+                drain_fwd_draw<MS>(a, n, ri, X, v0, sv, s_x, s_del, eb, s_c, s_q[q], fire);
             lds_wave_sync();
         }
     }
@@ -1359,7 +1368,7 @@ __device__ __forceinline__ StepView step_view(const RealInfo& ri, cptr<RealCtl> 
 template <int MS, int MR, bool EP, bool REDO>
 __device__ __forceinline__ void phase_b_tile(const KernelArgs& a, const NetView& n, uint32_t t,
                                              uint32_t (*s_x)[kTile], unsigned long long (*s_del)[kTile],
-                                             uint32_t* s_rp, int32_t* s_c, uint2* s_q) {
+                                             int32_t* s_c, uint2* s_q) {
     const uint32_t lane = threadIdx.x & 63u;
     const TileEnt te = ld_ro(a.tiles + t);
     const uint32_t r = wave_uniform(te.r);
@@ -1373,7 +1382,7 @@ __device__ __forceinline__ void phase_b_tile(const KernelArgs& a, const NetView&
     unsigned long long fire = 0;
     uint32_t bad = 0, viol = 0;
     if constexpr (EP) {
-        const bool complete = leap_tile_fwd<MS, MR, REDO>(a, n, ri, X, te.v0, sv, s_x, s_del, s_rp, s_c, s_q, fire);
+        const bool complete = leap_tile_fwd<MS, MR, REDO>(a, n, ri, X, te.v0, sv, s_x, s_del, s_c, s_q, fire);
         reduce_partB(fire, bad, viol);
         if (lane == 0) {
             if (!complete) {
@@ -1403,13 +1412,12 @@ void phase_b_kernel(KernelArgs a) {
     constexpr int LS = EP ? MS : 1;
     __shared__ uint32_t s_x[4][LS][kTile];
     __shared__ unsigned long long s_del[4][LS][kTile];
-    __shared__ uint32_t s_rp[4][kTile + 1];
     __shared__ int32_t s_c[4][kTile];
     __shared__ uint2 s_q[4][EP ? kQueue : 1];
     for (uint32_t it = 0;; ++it) {
         const uint32_t t = tile_at(it, a.ntiles, a.waves_per_block);
         if (t == 0xFFFFFFFFu) break;
-        phase_b_tile<MS, MR, EP, false>(a, n, t, s_x[wv], s_del[wv], s_rp[wv], s_c[wv], s_q[wv]);
+        phase_b_tile<MS, MR, EP, false>(a, n, t, s_x[wv], s_del[wv], s_c[wv], s_q[wv]);
     }
 }
 
@@ -1422,12 +1430,11 @@ __global__ __launch_bounds__(256) void phase_b_redo_kernel(KernelArgs a) {
     const uint32_t wpb = blockDim.x >> 6;
     __shared__ uint32_t s_x[4][MS][kTile];
     __shared__ unsigned long long s_del[4][MS][kTile];
-    __shared__ uint32_t s_rp[4][kTile + 1];
     __shared__ int32_t s_c[4][kTile];
     const uint32_t cnt = min(wave_uniform(*a.redo_count), a.ntiles);
     if (blockIdx.x == 0 && threadIdx.x == 0 && cnt != 0u) *a.redo_total += cnt;
     for (uint32_t q = blockIdx.x * wpb + wv; q < cnt; q += gridDim.x * wpb)
-        phase_b_tile<MS, MR, true, true>(a, n, wave_uniform(a.redo_tiles[q]), s_x[wv], s_del[wv], s_rp[wv], s_c[wv],
+        phase_b_tile<MS, MR, true, true>(a, n, wave_uniform(a.redo_tiles[q]), s_x[wv], s_del[wv], s_c[wv],
                                          nullptr);
 }
 

@@ -112,6 +112,7 @@ struct KernelArgs {
     const uint32_t* frp;
     const uint32_t* fcol;
     const float* fw;
+    const uint8_t* fown;  // per forward entry: its row's index within the row's tile
     unsigned long long* gdelta;
     unsigned long long* redo_total;  // tiles redone since the context was made
     double epsilon;
