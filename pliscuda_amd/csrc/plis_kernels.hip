@@ -1401,10 +1401,12 @@ __device__ __forceinline__ void phase_b_tile(const KernelArgs& a, const NetView&
     if (lane == 0) a.partB_tile[t] = PartB{fire, bad, viol};
 }
 
-// The one-compartment edge-parallel leap of up to 5 species fits in 96
-// VGPRs without spills; asking for 5 waves per SIMD keeps it there.
+// The one-compartment edge-parallel leap of up to 5 species: asking for 6
+// waves per SIMD holds it to 80 VGPRs (16 bytes of spill per lane), which
+// measures faster than 85 VGPRs at 5 waves (the walk waits on gathers and
+// LDS, more waves hide it).
 template <int MS, int MR, bool EP, bool MC>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EP && !MC && MS <= 5 ? 5 : 1)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EP && !MC && MS <= 5 ? 6 : 1)))
 void phase_b_kernel(KernelArgs a) {
     const NetView n = net_view<MC>(a);
     const uint32_t wv = wave_uniform(threadIdx.x >> 6);
