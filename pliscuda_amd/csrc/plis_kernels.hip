@@ -424,7 +424,8 @@ __device__ __forceinline__ void tau_in_edge(int S, const double (&du)[MS], const
 #pragma unroll
     for (int s = 0; s < MS; ++s) {
         if (s >= S) break;
-        const uint32_t xe = xu[s] >= nc1 ? xu[s] : 0u;
+        uint32_t xe = xu[s] >= nc1 ? xu[s] : 0u;
+        asm("" : "+v"(xe));  // select the count, not its conversion (one select instead of two)
         const double ain = (du[s] * (double)xe) * wq;
         t.mu[s] += ain;
         t.sig[s] += ain;
@@ -577,10 +578,15 @@ __device__ __forceinline__ void cao_tau_tile(const KernelArgs& a, const NetView&
 
 template <int MS, int MR, bool MC>
 __global__ __launch_bounds__(256) void phase_a_kernel(KernelArgs a) {
-    const NetView n = net_view<MC>(a);
     for (uint32_t it = 0;; ++it) {
         const uint32_t t = tile_at(it, a.ntiles, a.waves_per_block);
         if (t == 0xFFFFFFFFu) break;
+        // the network tables are read through pointers made opaque per tile,
+        // so their scalar loads stay inside the tile's work instead of being
+        // hoisted out of the tile loop into (spilled) scalar registers
+        KernelArgs al = a;
+        asm volatile("" : "+s"(al.net_i), "+s"(al.net_d));
+        const NetView n = net_view<MC>(al);
         const TileEnt te = ld_ro(a.tiles + t);
         const uint32_t r = wave_uniform(te.r);
         const RealInfo ri = ld_ro(a.info + r);
@@ -1408,7 +1414,6 @@ __device__ __forceinline__ void phase_b_tile(const KernelArgs& a, const NetView&
 template <int MS, int MR, bool EP, bool MC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EP && !MC && MS <= 5 ? 6 : 1)))
 void phase_b_kernel(KernelArgs a) {
-    const NetView n = net_view<MC>(a);
     const uint32_t wv = wave_uniform(threadIdx.x >> 6);
     // LDS staging for the edge-parallel path (up to 4 waves per block)
     constexpr int LS = EP ? MS : 1;
@@ -1419,6 +1424,9 @@ void phase_b_kernel(KernelArgs a) {
     for (uint32_t it = 0;; ++it) {
         const uint32_t t = tile_at(it, a.ntiles, a.waves_per_block);
         if (t == 0xFFFFFFFFu) break;
+        KernelArgs al = a;  // network tables through per-tile opaque pointers, as in phase_a_kernel
+        asm volatile("" : "+s"(al.net_i), "+s"(al.net_d));
+        const NetView n = net_view<MC>(al);
         phase_b_tile<MS, MR, EP, false>(a, n, t, s_x[wv], s_del[wv], s_c[wv], s_q[wv]);
     }
 }
