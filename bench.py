@@ -137,29 +137,41 @@ def gpu_measure(wl, steps, warmup, seed=2024, autotune=True, shape=None):
 
 
 def learned_measure(wl, steps, warmup, train_steps, batch, seed=2024, kappa=1.0, snapshots=16,
-                    per_snapshot=16384, shape=None, act=1, records=None):
+                    per_snapshot=16384, shape=None, act=1, records=None, net=None):
     """Records from plain steps of the same workload, GPU training, then the
     learned step timed like the plain one.  act: the network's activation
-    (1 GELU, 0 ReLU); records: (F, Y, L, plain_stats) to reuse."""
+    (1 GELU, 0 ReLU); records: (F, Y, L, plain_stats) to reuse; net: a
+    trained network {"w", "norm", "vmax"} to use instead of records and
+    training."""
     import torch
 
     from pliscuda_amd import capi, learned
-    rec = capi.Context(wl["net"], wl["topos"], wl["x0"], eps=wl["eps"], nc=wl["nc"], seed=seed + 1)
-    t0 = time.perf_counter()
-    if records is None:
-        F, Y, Cc = learned.collect_records(rec, snapshots=snapshots, stride=4, per_snapshot=per_snapshot)
-        plain_stats = [rec.stats(r) for r in range(rec.n)]
+    t_rec = t_train = 0.0
+    info = {}
+    if net is None:
+        rec = capi.Context(wl["net"], wl["topos"], wl["x0"], eps=wl["eps"], nc=wl["nc"], seed=seed + 1)
+        t0 = time.perf_counter()
+        if records is None:
+            F, Y, Cc = learned.collect_records(rec, snapshots=snapshots, stride=4, per_snapshot=per_snapshot)
+            plain_stats = [rec.stats(r) for r in range(rec.n)]
+        else:
+            F, Y, Cc, plain_stats = records
+        t_rec = time.perf_counter() - t0
+        vmax = learned.calibrate_viol_rate(plain_stats, [int(t["N"]) for t in wl["topos"]])
+        rec.mlp_activation(act)
+        t0 = time.perf_counter()
+        norm, losses = learned.train(rec, F, Y, Cc, steps=train_steps, batch=batch, lr=1e-3)
+        rec.sync()
+        t_train = time.perf_counter() - t0
+        w = rec.mlp_get()
+        rec.close()
+        info = {"records": int(F.shape[0]), "reject_label_rate": float(Cc.mean()), "_records": (F, Y, Cc, plain_stats),
+                "train_steps": train_steps, "train_batch": batch,
+                "loss_first": float(losses[:10].mean()), "loss_last": float(losses[-100:].mean()),
+                "_net": {"w": w, "norm": norm, "vmax": vmax}}
     else:
-        F, Y, Cc, plain_stats = records
-    t_rec = time.perf_counter() - t0
-    vmax = learned.calibrate_viol_rate(plain_stats, [int(t["N"]) for t in wl["topos"]])
-    rec.mlp_activation(act)
-    t0 = time.perf_counter()
-    norm, losses = learned.train(rec, F, Y, Cc, steps=train_steps, batch=batch, lr=1e-3)
-    rec.sync()
-    t_train = time.perf_counter() - t0
-    w = rec.mlp_get()
-    rec.close()
+        w, norm, vmax = net["w"], net["norm"], float(net["vmax"])
+        info = {"network": "loaded"}
     wpb, bpc = shape if shape else (0, 0)
     ctx = capi.Context(wl["net"], wl["topos"], wl["x0"], eps=wl["eps"], nc=wl["nc"], seed=seed,
                        waves_per_block=wpb, blocks_per_cu=bpc)
@@ -192,11 +204,8 @@ def learned_measure(wl, steps, warmup, train_steps, batch, seed=2024, kappa=1.0,
             "tau_ms": prof[1] / n_prof, "decide_ms": prof[2] / n_prof, "leap_ms": prof[3] / n_prof,
             "commit_ms": prof[4] / n_prof, "leap_kernel_ms": prof[5] / n_prof, "redo_ms": prof[6] / n_prof,
             "finish_with_proposals_ms": prof[7] / n_prof, "iteration_ms_events": prof[0] / n_prof,
-            "records": int(F.shape[0]), "reject_label_rate": float(Cc.mean()), "records_seconds": t_rec,
-            "train_steps": train_steps, "_records": (F, Y, Cc, plain_stats),
-            "train_batch": batch, "train_seconds": t_train,
-            "loss_first": float(losses[:10].mean()), "loss_last": float(losses[-100:].mean()),
-            "viol_rate_max": vmax, "kappa": kappa}
+            "records_seconds": t_rec, "train_seconds": t_train,
+            "viol_rate_max": vmax, "kappa": kappa, **info}
 
 
 def cpu_baseline(wl, seed, sample_steps, gpu_check=True):
@@ -361,6 +370,9 @@ def main():
     ap.add_argument("--train-steps", type=int, default=20000)
     ap.add_argument("--train-batch", type=int, default=65536)
     ap.add_argument("--kappa", type=float, default=1.0)
+    ap.add_argument("--net-file", default=None,
+                    help="trained networks of the learned legs (.npz): loaded when the file exists "
+                         "(no records, no training), else written after training")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
@@ -441,11 +453,22 @@ def main():
     if not args.no_learned and args.workload == "brusselator":
         simt_plain = res["subvolume_simtime"] / res["seconds"]
         recs = None
+        nets = {}
+        if args.net_file and os.path.exists(args.net_file):
+            with np.load(args.net_file) as z:
+                nets = {a: {"w": z[f"w{a}"], "norm": z[f"norm{a}"], "vmax": z[f"vmax{a}"]} for a in (0, 1)}
         # the network as specified (GELU), then the same records with ReLU
         for key, act in (("learned", 1), ("learned_relu", 0)):
             lr_ = learned_measure(wl, args.steps, args.warmup, args.train_steps, args.train_batch,
-                                  kappa=args.kappa, shape=res["tuned"], act=act, records=recs)
-            recs = lr_.pop("_records")
+                                  kappa=args.kappa, shape=res["tuned"], act=act, records=recs,
+                                  net=nets.get(act))
+            recs = lr_.pop("_records", None)
+            if "_net" in lr_:
+                nets_new = lr_.pop("_net")
+                if args.net_file:
+                    nets[act] = nets_new
+            if args.net_file and len(nets) == 2 and not os.path.exists(args.net_file):
+                np.savez(args.net_file, **{f"{k}{a}": np.asarray(v) for a, d in nets.items() for k, v in d.items()})
             lv = lr_["subvolume_leaps"] / lr_["seconds"]
             simt_learned = lr_["subvolume_simtime"] / lr_["seconds"]
             out[key] = {"subvolume_leaps_per_s": lv, "firings_per_s": lr_["firings"] / lr_["seconds"],
