@@ -1,6 +1,7 @@
 """Statistical correctness of the tau-leap step (oracle on the CPU; the GPU
-is bit-identical to it, and tests/test_gpu_statistics.py repeats the main
-comparisons through the HIP path and compares the learned mode with it).
+is bit-identical to it -- tests/test_gpu_parity.py and
+tests/test_configs_gpu.py -- and tests/test_mlp_gpu.py compares the learned
+mode's ensemble statistics with the plain method's on the GPU).
 
 * closed forms: birth-death stationary Poisson law, first-order decay
   (binomial), mean of pure diffusion e^{tQ} x0;
