@@ -220,3 +220,44 @@ def test_poisson_goodness_of_fit(lam):
     chi2 = ((obs - exp) ** 2 / exp).sum()
     p = stats.chi2.sf(chi2, keep.sum() - 1)
     assert p > 1e-4, (lam, chi2, p)
+
+
+def _coarse_bound(lf):
+    """The leap walk's bound on a draw's top byte (float32 operations in the
+    kernel's order): (255 - 2^-12) - lf (256 + 2^-10) for lf > 0."""
+    lf = np.float32(lf)
+    return np.where(lf > 0, np.float32(255.0 - 2.0 ** -12) - lf * np.float32(256.0 + 2.0 ** -10),
+                    np.float32(256.0)).astype(np.float32)
+
+
+def test_coarse_bound_settles_only_zero_draws():
+    """A draw whose top byte hb satisfies hb <= coarse_bound(lf) is settled as
+    0 without its fine bits; that is sound if every word with that byte gives
+    0 for every double mean the float mean lf can stand for (lf within 8 2^-24
+    relative).  Over 200,000 float means (log-uniform 1e-9..2, and dense just
+    below each byte's threshold) and every byte: (hb + 1) 2^-8 <= 1 - lam_max
+    holds wherever the bound settles, and the oracle's inversion of the
+    largest word with that byte at lam_max is 0 on a sample; the bound also
+    settles almost every draw it can (it gives away at most one byte value
+    against the exact threshold)."""
+    rng = np.random.default_rng(5)
+    lf = np.concatenate([10.0 ** rng.uniform(-9, np.log10(2.0), 150000),
+                         # means right at the byte thresholds 1 - (hb + 1)/256
+                         (1.0 - (rng.integers(0, 256, 50000) + 1) / 256.0) * (1 + rng.uniform(-1e-6, 1e-6, 50000))])
+    lf = lf[lf > 0].astype(np.float32)
+    g = _coarse_bound(lf)
+    hb = np.arange(256, dtype=np.float32)
+    settled = hb[None, :] <= g[:, None]
+    lam_max = lf.astype(np.float64) * (1.0 + 8 * 2.0 ** -24)
+    sound = (hb[None, :].astype(np.float64) + 1.0) / 256.0 <= 1.0 - lam_max[:, None]
+    assert not (settled & ~sound).any()
+    # the exact threshold settles hb iff (hb + 1)/256 <= 1 - lam for the float mean itself
+    exact = (hb[None, :].astype(np.float64) + 1.0) / 256.0 <= 1.0 - lf.astype(np.float64)[:, None]
+    assert ((exact.sum(1) - settled.sum(1)) <= 1).all()
+    i, j = np.nonzero(settled)
+    pick = rng.choice(len(i), 2000, replace=False)
+    for a, b in zip(i[pick], j[pick]):
+        word = (int(b) << 24) | 0xFFFFFF
+        assert ffi.poisson_inversion(float(lam_max[a]), word) == 0
+    # a mean above 1 is never settled
+    assert not settled[lf > 1.0].any()
