@@ -576,8 +576,12 @@ __device__ __forceinline__ void cao_tau_tile(const KernelArgs& a, const NetView&
     if (lane == 0) a.partA_tile[t] = PartA{tp, t2, v2, 0u};
 }
 
+// The one-compartment tau pass of up to 5 species fits in 64 VGPRs without
+// spills; asking for 8 waves per SIMD (70 VGPRs and 7 waves otherwise)
+// hides more of its gathers' latency.
 template <int MS, int MR, bool MC>
-__global__ __launch_bounds__(256) void phase_a_kernel(KernelArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(!MC && MS <= 5 ? 8 : 1)))
+void phase_a_kernel(KernelArgs a) {
     for (uint32_t it = 0;; ++it) {
         const uint32_t t = tile_at(it, a.ntiles, a.waves_per_block);
         if (t == 0xFFFFFFFFu) break;
