@@ -852,7 +852,9 @@ static int run_one(const model_t *m, const csr_t *g, const orc_params *p, uint32
         st.tau_p = tpg; st.tau2 = t2g; st.vstar = vs; st.attempt = 0;
         st.ev_kind = 0;
         decide(&st);
-        if (t2g < INFINITY) choose_event(m, g, p, x, real, &st);
+        /* needed only if tau'' <= tau': a leap-back halves tau' (and may lift
+         * the clamp to t_end) but never makes a larger tau'' fire */
+        if (t2g < INFINITY && t2g <= tpg) choose_event(m, g, p, x, real, &st);
         for (;;) {
             /* phase B */
 #pragma omp parallel for schedule(static) if (par)

@@ -542,7 +542,9 @@ __device__ void finalize_A(const KernelArgs& a, const NetView& n, const RealInfo
     c.ev_idx = -1;
     c.ev_dst = v2;
     decide(c);
-    if (t2 < kInf) choose_event<MS, MR>(a, n, ri, c, c.cur ? a.x1 : a.x0);
+    // the event is needed only if tau'' <= tau' (a leap-back halves tau' and
+    // can lift the clamp to t_end, but never makes a larger tau'' fire)
+    if (t2 < kInf && t2 <= tp) choose_event<MS, MR>(a, n, ri, c, c.cur ? a.x1 : a.x0);
     if (lead) a.ctl[r] = c;
 }
 
