@@ -522,9 +522,8 @@ __device__ __forceinline__ void reduce_partA(double& tp, double& t2, uint32_t& v
 
 template <int MS, int MR>
 __device__ void finalize_A(const KernelArgs& a, const NetView& n, const RealInfo& ri, uint32_t r,
-                           double tp, double t2, uint32_t v2) {
+                           double tp, double t2, uint32_t v2, RealCtl c) {
     const bool lead = (threadIdx.x & 63u) == 0u;
-    RealCtl c = a.ctl[r];
     if (tp == kInf && t2 == kInf) {
         c.status = kAbsorbed;
         c.done = 1u;
@@ -617,8 +616,10 @@ __global__ __launch_bounds__(decide_threads<MS>()) void decide_kernel(KernelArgs
     const uint32_t r = blockIdx.x;
     if (r == 0 && threadIdx.x == 0) *a.redo_count = 0u;  // the coming leap pass's redo list
     if (r >= a.nreal) return;
-    const RealInfo ri = a.info[r];
-    if (a.ctl[r].done || a.ctl[r].retry) return;
+    // the realization's control block and info in one round of loads
+    const RealInfo ri = ld_ro(a.info + r);
+    const RealCtl c0 = ld_ro(a.ctl + r);
+    if (c0.done || c0.retry) return;
     __shared__ double s_tp[16], s_t2[16];
     __shared__ uint32_t s_v2[16];
     double tp = kInf, t2 = kInf;
@@ -656,7 +657,7 @@ __global__ __launch_bounds__(decide_threads<MS>()) void decide_kernel(KernelArgs
                 v2 = s_v2[w];
             }
         }
-        finalize_A<MS, MR>(a, net_view(a), ri, r, tp, t2, v2);
+        finalize_A<MS, MR>(a, net_view(a), ri, r, tp, t2, v2, c0);
     }
 }
 
@@ -1301,8 +1302,7 @@ __device__ __forceinline__ void reduce_partB(unsigned long long& fire, uint32_t&
 }
 
 __device__ void finalize_B(const KernelArgs& a, const RealInfo& ri, uint32_t r, unsigned long long fire,
-                           uint32_t bad, uint32_t viol) {
-    RealCtl c = a.ctl[r];
+                           uint32_t bad, uint32_t viol, RealCtl c) {
     // A learned proposal is judged by Cao's leap condition after the fact: it
     // is kept only without negative populations and with no more violating
     // subvolumes than the plain method produces -- mean mu = viol_rate_max N,
@@ -1486,8 +1486,9 @@ __global__ __launch_bounds__(256) void phase_b_finish_kernel(KernelArgs a) {
 __global__ __launch_bounds__(1024) void commit_kernel(KernelArgs a) {
     const uint32_t r = blockIdx.x;
     if (r >= a.nreal) return;
-    const RealInfo ri = a.info[r];
-    if (a.ctl[r].done) return;
+    const RealInfo ri = ld_ro(a.info + r);
+    const RealCtl c0 = ld_ro(a.ctl + r);
+    if (c0.done) return;
     __shared__ unsigned long long s_f[16];
     __shared__ uint32_t s_b[16], s_v[16];
     unsigned long long fire = 0;
@@ -1520,7 +1521,7 @@ __global__ __launch_bounds__(1024) void commit_kernel(KernelArgs a) {
             bad |= s_b[w];
             viol += s_v[w];
         }
-        finalize_B(a, ri, r, fire, bad, viol);
+        finalize_B(a, ri, r, fire, bad, viol, c0);
     }
 }
 
