@@ -86,6 +86,8 @@ struct plis_ctx {
     float *d_w = nullptr, *d_iw = nullptr;
     int32_t* d_comp = nullptr;
     PartA* d_pa_t = nullptr;
+    PartA* d_pa_c = nullptr;      // decide_kernel's chunk partials
+    uint32_t* d_dcnt = nullptr;   // decide_kernel's per-realization chunk counters
     PartB* d_pb_t = nullptr;
     uint32_t* d_cnt = nullptr;   // [0] realizations done, [1] redo-list length, [2..3] redone tiles (u64)
     uint32_t* d_redo = nullptr;  // redo list of the leap pass (tile ids)
@@ -322,6 +324,8 @@ static KernelArgs make_args(plis_ctx* c) {
     a.iw = c->d_iw;
     a.comp = c->d_comp;
     a.partA_tile = c->d_pa_t;
+    a.partA_chunk = c->d_pa_c;
+    a.decide_cnt = c->d_dcnt;
     a.partB_tile = c->d_pb_t;
     a.ndone = c->d_cnt;
     a.redo_count = c->d_cnt + 1;
@@ -361,6 +365,7 @@ static const int kVariantR[] = {4, 4, 8, 16, 128};
 template <int MS, int MR>
 static void launch_variant(plis_ctx* c, int which, const KernelArgs& a) {
     const dim3 grid(c->blocks), block(64 * c->wpb), rgrid(c->nreal), rblock(1024), dblock(decide_threads<MS>());
+    const dim3 dgrid(c->nreal * kDecideChunks);
     const bool mc = c->nh.C > 1;  // several compartments: per-subvolume rate tables
     const bool gelu = c->mlp_act == 1;
     // learned kernels stage the network in LDS once per block: fewer, longer-lived blocks
@@ -385,7 +390,7 @@ static void launch_variant(plis_ctx* c, int which, const KernelArgs& a) {
             else hipLaunchKernelGGL((phase_a_kernel<MS, MR, false>), grid, block, 0, c->stream, a);
             break;
         case 1:
-            hipLaunchKernelGGL((decide_kernel<MS, MR>), rgrid, dblock, 0, c->stream, a);
+            hipLaunchKernelGGL((decide_kernel<MS, MR>), dgrid, dblock, 0, c->stream, a);
             break;
         case 2:
             if constexpr (MS <= 8) {
@@ -706,6 +711,8 @@ extern "C" int plis_create(const plis_network* net, int32_t n_real, const plis_t
         (rc = upload(c, &c->d_iw, hiw)) || (rc = upload(c, &c->d_comp, hcomp)) ||
         (rc = dev_alloc(c, &c->d_pa_t, c->ntiles)) || (rc = dev_alloc(c, &c->d_pb_t, c->ntiles)) ||
         (rc = dev_alloc(c, &c->d_redo, c->ntiles)) ||
+        (rc = dev_alloc(c, &c->d_pa_c, (size_t)c->nreal * kDecideChunks)) ||
+        (rc = upload(c, &c->d_dcnt, std::vector<uint32_t>(c->nreal, 0u))) ||
         (rc = upload(c, &c->d_cnt, zeros_cnt))) {
         g_create_err = c->err;
         plis_destroy(c);
@@ -1088,12 +1095,15 @@ extern "C" int plis_launch_shape(plis_ctx* c, int32_t* out) {
 // non-current count buffers.
 constexpr uint32_t kPoisonCount = 0x7FFFFFF0u;
 
-__global__ __launch_bounds__(256) void poison_parts_kernel(PartA* pa, PartB* pb, uint32_t* redo, uint32_t ntiles) {
+__global__ __launch_bounds__(256) void poison_parts_kernel(PartA* pa, PartB* pb, uint32_t* redo, uint32_t ntiles,
+                                                           PartA* pc, uint32_t nchunks) {
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < ntiles; t += gridDim.x * blockDim.x) {
         pa[t] = PartA{1e-300, 1e-300, 0u, 0xDEADu};
         pb[t] = PartB{1ull << 40, 1u, 7u};
         redo[t] = 0u;
     }
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nchunks; t += gridDim.x * blockDim.x)
+        pc[t] = PartA{1e-300, 1e-300, 0u, 0xDEADu};
 }
 
 extern "C" int plis_debug_poison(plis_ctx* c) {
@@ -1105,7 +1115,8 @@ extern "C" int plis_debug_poison(plis_ctx* c) {
         HIPCHK(c, hipMemsetD32Async(other + ri.x_off, (int)kPoisonCount, (size_t)c->S * ri.N, c->stream));
     }
     hipLaunchKernelGGL(poison_parts_kernel, dim3(std::min<uint32_t>((c->ntiles + 255) / 256, 1024u)), dim3(256), 0,
-                       c->stream, c->d_pa_t, c->d_pb_t, c->d_redo, c->ntiles);
+                       c->stream, c->d_pa_t, c->d_pb_t, c->d_redo, c->ntiles, c->d_pa_c,
+                       c->nreal * kDecideChunks);
     HIPCHK(c, hipGetLastError());
     // the partials no longer hold any proposal
     return invalidate_props(c);

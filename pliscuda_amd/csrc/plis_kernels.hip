@@ -611,36 +611,18 @@ void phase_a_kernel(KernelArgs a) {
 template <int MS>
 constexpr uint32_t decide_threads() { return MS <= 5 ? 1024u : 256u; }
 
-template <int MS, int MR>
-__global__ __launch_bounds__(decide_threads<MS>()) void decide_kernel(KernelArgs a) {
-    const uint32_t r = blockIdx.x;
-    if (r == 0 && threadIdx.x == 0) *a.redo_count = 0u;  // the coming leap pass's redo list
-    if (r >= a.nreal) return;
-    // the realization's control block and info in one round of loads
-    const RealInfo ri = ld_ro(a.info + r);
-    const RealCtl c0 = ld_ro(a.ctl + r);
-    if (c0.done || c0.retry) return;
-    __shared__ double s_tp[16], s_t2[16];
-    __shared__ uint32_t s_v2[16];
-    double tp = kInf, t2 = kInf;
-    uint32_t v2 = 0xFFFFFFFFu;
-    // 8 independent loads in flight per thread
-    for (uint32_t q0 = 0; q0 < ri.ntiles; q0 += 8u * blockDim.x) {
-        PartA p[8];
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const uint32_t q = q0 + (uint32_t)i * blockDim.x + threadIdx.x;
-            p[i] = q < ri.ntiles ? a.partA_tile[ri.tile_begin + q] : PartA{kInf, kInf, 0xFFFFFFFFu, 0u};
-        }
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            tp = fmin(tp, p[i].tp);
-            if (p[i].t2 < t2 || (p[i].t2 == t2 && p[i].v2 < v2)) {
-                t2 = p[i].t2;
-                v2 = p[i].v2;
-            }
-        }
-    }
+// decide_kernel runs kDecideChunks blocks per realization: each reduces one
+// chunk of the realization's tile partials (the reads spread over as many
+// CUs), publishes its partial and counts itself in; the last block of the
+// realization reduces the chunk partials and decides the step.  Release
+// fence before the count, acquire fence after it (agent scope: the blocks
+// run on different XCDs); the last block clears the counter for the next
+// step (the next launch is ordered after this kernel).
+constexpr uint32_t kDecideChunks = 16;
+
+template <int MS>
+__device__ __forceinline__ void reduce_block_partA(double& tp, double& t2, uint32_t& v2, double* s_tp, double* s_t2,
+                                                   uint32_t* s_v2) {
     reduce_partA(tp, t2, v2);
     const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     if (lane == 0) {
@@ -657,8 +639,61 @@ __global__ __launch_bounds__(decide_threads<MS>()) void decide_kernel(KernelArgs
                 v2 = s_v2[w];
             }
         }
-        finalize_A<MS, MR>(a, net_view(a), ri, r, tp, t2, v2, c0);
     }
+}
+
+template <int MS, int MR>
+__global__ __launch_bounds__(decide_threads<MS>()) void decide_kernel(KernelArgs a) {
+    const uint32_t r = blockIdx.x / kDecideChunks, chunk = blockIdx.x % kDecideChunks;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *a.redo_count = 0u;  // the coming leap pass's redo list
+    if (r >= a.nreal) return;
+    // the realization's control block and info in one round of loads
+    const RealInfo ri = ld_ro(a.info + r);
+    const RealCtl c0 = ld_ro(a.ctl + r);
+    if (c0.done || c0.retry) return;  // the same in every block of the realization
+    __shared__ double s_tp[16], s_t2[16];
+    __shared__ uint32_t s_v2[16];
+    __shared__ uint32_t s_last;
+    const uint32_t per = (ri.ntiles + kDecideChunks - 1u) / kDecideChunks;
+    const uint32_t qb = min(chunk * per, ri.ntiles), qe = min(qb + per, ri.ntiles);
+    double tp = kInf, t2 = kInf;
+    uint32_t v2 = 0xFFFFFFFFu;
+    // 8 independent loads in flight per thread
+    for (uint32_t q0 = qb; q0 < qe; q0 += 8u * blockDim.x) {
+        PartA p[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const uint32_t q = q0 + (uint32_t)i * blockDim.x + threadIdx.x;
+            p[i] = q < qe ? a.partA_tile[ri.tile_begin + q] : PartA{kInf, kInf, 0xFFFFFFFFu, 0u};
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            tp = fmin(tp, p[i].tp);
+            if (p[i].t2 < t2 || (p[i].t2 == t2 && p[i].v2 < v2)) {
+                t2 = p[i].t2;
+                v2 = p[i].v2;
+            }
+        }
+    }
+    reduce_block_partA<MS>(tp, t2, v2, s_tp, s_t2, s_v2);
+    if (threadIdx.x == 0) {
+        a.partA_chunk[r * kDecideChunks + chunk] = PartA{tp, t2, v2, 0u};
+        __threadfence();
+        s_last = atomicAdd(&a.decide_cnt[r], 1u) == kDecideChunks - 1u ? 1u : 0u;
+    }
+    __syncthreads();
+    if (!s_last || threadIdx.x >= 64u) return;
+    // the last block of the realization: its first wavefront decides
+    __threadfence();
+    const uint32_t lane = threadIdx.x;
+    PartA p{kInf, kInf, 0xFFFFFFFFu, 0u};
+    if (lane < kDecideChunks) p = a.partA_chunk[r * kDecideChunks + lane];
+    tp = p.tp;
+    t2 = p.t2;
+    v2 = p.v2;
+    reduce_partA(tp, t2, v2);
+    if (lane == 0) a.decide_cnt[r] = 0u;
+    finalize_A<MS, MR>(a, net_view(a), ri, r, tp, t2, v2, c0);
 }
 
 // ---------------------------------------------------------------------------

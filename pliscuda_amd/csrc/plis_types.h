@@ -103,6 +103,8 @@ struct KernelArgs {
     const float* iw;
     const int32_t* comp;
     PartA* partA_tile;
+    PartA* partA_chunk;      // decide_kernel: one partial per (realization, chunk of its tiles)
+    uint32_t* decide_cnt;    // decide_kernel: chunks done per realization (zero between steps)
     PartB* partB_tile;
     uint32_t* ndone;
     uint32_t* redo_count;    // tiles phase B left to the redo pass (reset by decide)
