@@ -577,11 +577,12 @@ __device__ __forceinline__ void cao_tau_tile(const KernelArgs& a, const NetView&
     if (lane == 0) a.partA_tile[t] = PartA{tp, t2, v2, 0u};
 }
 
-// The one-compartment tau pass of up to 5 species fits in 64 VGPRs without
+// The one-compartment tau pass of 3 to 5 species fits in 64 VGPRs without
 // spills; asking for 8 waves per SIMD (70 VGPRs and 7 waves otherwise)
-// hides more of its gathers' latency.
+// hides more of its gathers' latency (Brusselator mesh 77.6 -> 74.8 us; the
+// one-species instantiation measured slower with it, 11.3 -> 12.4 us).
 template <int MS, int MR, bool MC>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(!MC && MS <= 5 ? 8 : 1)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(!MC && MS >= 3 && MS <= 5 ? 8 : 1)))
 void phase_a_kernel(KernelArgs a) {
     for (uint32_t it = 0;; ++it) {
         const uint32_t t = tile_at(it, a.ntiles, a.waves_per_block);
@@ -611,13 +612,15 @@ void phase_a_kernel(KernelArgs a) {
 template <int MS>
 constexpr uint32_t decide_threads() { return MS <= 5 ? 1024u : 256u; }
 
-// decide_kernel runs kDecideChunks blocks per realization: each reduces one
-// chunk of the realization's tile partials (the reads spread over as many
-// CUs), publishes its partial and counts itself in; the last block of the
-// realization reduces the chunk partials and decides the step.  Release
-// fence before the count, acquire fence after it (agent scope: the blocks
-// run on different XCDs); the last block clears the counter for the next
-// step (the next launch is ordered after this kernel).
+// decide_kernel runs decide_chunks blocks per realization (one per 1,024
+// tile partials of the largest realization, at most kDecideChunks): each
+// reduces one chunk of the realization's tile partials (the reads spread
+// over as many CUs), publishes its partial and counts itself in; the last
+// block of the realization reduces the chunk partials and decides the
+// step.  Release fence before the count, acquire fence after it (agent
+// scope: the blocks run on different XCDs); the last block clears the
+// counter for the next step (the next launch is ordered after this
+// kernel).  With one block per realization it decides directly.
 constexpr uint32_t kDecideChunks = 16;
 
 template <int MS>
@@ -644,7 +647,8 @@ __device__ __forceinline__ void reduce_block_partA(double& tp, double& t2, uint3
 
 template <int MS, int MR>
 __global__ __launch_bounds__(decide_threads<MS>()) void decide_kernel(KernelArgs a) {
-    const uint32_t r = blockIdx.x / kDecideChunks, chunk = blockIdx.x % kDecideChunks;
+    const uint32_t nch = a.decide_chunks;
+    const uint32_t r = blockIdx.x / nch, chunk = blockIdx.x % nch;
     if (blockIdx.x == 0 && threadIdx.x == 0) *a.redo_count = 0u;  // the coming leap pass's redo list
     if (r >= a.nreal) return;
     // the realization's control block and info in one round of loads
@@ -654,7 +658,7 @@ __global__ __launch_bounds__(decide_threads<MS>()) void decide_kernel(KernelArgs
     __shared__ double s_tp[16], s_t2[16];
     __shared__ uint32_t s_v2[16];
     __shared__ uint32_t s_last;
-    const uint32_t per = (ri.ntiles + kDecideChunks - 1u) / kDecideChunks;
+    const uint32_t per = (ri.ntiles + nch - 1u) / nch;
     const uint32_t qb = min(chunk * per, ri.ntiles), qe = min(qb + per, ri.ntiles);
     double tp = kInf, t2 = kInf;
     uint32_t v2 = 0xFFFFFFFFu;
@@ -676,10 +680,14 @@ __global__ __launch_bounds__(decide_threads<MS>()) void decide_kernel(KernelArgs
         }
     }
     reduce_block_partA<MS>(tp, t2, v2, s_tp, s_t2, s_v2);
+    if (nch == 1u) {
+        if (threadIdx.x < 64u) finalize_A<MS, MR>(a, net_view(a), ri, r, tp, t2, v2, c0);
+        return;
+    }
     if (threadIdx.x == 0) {
-        a.partA_chunk[r * kDecideChunks + chunk] = PartA{tp, t2, v2, 0u};
+        a.partA_chunk[r * nch + chunk] = PartA{tp, t2, v2, 0u};
         __threadfence();
-        s_last = atomicAdd(&a.decide_cnt[r], 1u) == kDecideChunks - 1u ? 1u : 0u;
+        s_last = atomicAdd(&a.decide_cnt[r], 1u) == nch - 1u ? 1u : 0u;
     }
     __syncthreads();
     if (!s_last || threadIdx.x >= 64u) return;
@@ -687,7 +695,7 @@ __global__ __launch_bounds__(decide_threads<MS>()) void decide_kernel(KernelArgs
     __threadfence();
     const uint32_t lane = threadIdx.x;
     PartA p{kInf, kInf, 0xFFFFFFFFu, 0u};
-    if (lane < kDecideChunks) p = a.partA_chunk[r * kDecideChunks + lane];
+    if (lane < nch) p = a.partA_chunk[r * nch + lane];
     tp = p.tp;
     t2 = p.t2;
     v2 = p.v2;

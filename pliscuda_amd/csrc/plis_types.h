@@ -105,6 +105,7 @@ struct KernelArgs {
     PartA* partA_tile;
     PartA* partA_chunk;      // decide_kernel: one partial per (realization, chunk of its tiles)
     uint32_t* decide_cnt;    // decide_kernel: chunks done per realization (zero between steps)
+    uint32_t decide_chunks;  // decide_kernel: blocks per realization (1 .. kDecideChunks)
     PartB* partB_tile;
     uint32_t* ndone;
     uint32_t* redo_count;    // tiles phase B left to the redo pass (reset by decide)
