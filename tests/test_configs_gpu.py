@@ -8,7 +8,8 @@ properties the configuration fixes where it does not.
   bit-exact, then the run to t = 5 ends exactly at t_end with counts between
   the model's two stable states' ranges.
 * Brusselator 128x128x64 random mesh (1,048,576 subvolumes, ~8.0 M directed
-  edges): tau pass bit-exact, 10 steps bit-exact.
+  edges): tau pass bit-exact, 10 steps bit-exact; the same mesh batched with
+  two small meshes, 8 steps of each bit-exact.
 * 256 Schlogl cubes of side 16..48 in one context: 40 steps of the smallest,
   the largest and six more realizations bit-exact against the oracle, then the
   whole batch to t = 5.
@@ -92,6 +93,28 @@ def test_brusselator_mesh_1M():
     assert np.array_equal(ctx.get_state(), x), "state after 10 steps differs"
     _same_stats(ctx.stats(), st, "10 steps")
     assert st.firings > 10 * 300_000
+    ctx.close()
+
+
+def test_mesh_with_small_realizations_in_one_context():
+    """The 1,048,576-subvolume mesh (16,384 tiles: the step decision reduces
+    its tile partials in 16 chunk blocks, the last of which decides) in one
+    context with two small meshes of the same network (one tile each, whose
+    chunk blocks but the first find no partials): 8 steps of every
+    realization bit-identical to the oracle's single runs."""
+    wl = W.brusselator_mesh()
+    net = wl["net"]
+    small = [W.random_mesh(4, 4, 4, 5), W.random_mesh(8, 4, 2, 6)]
+    topos = [wl["topos"][0]] + small
+    x0s = [wl["x0"][0]] + [W.poisson_state(5, t["N"], [60, 40, 80, 100, 20], 7 + i) for i, t in enumerate(small)]
+    ctx = capi.Context(net, topos, x0s, eps=wl["eps"], nc=wl["nc"], seed=SEED)
+    ctx.run(np.inf, step_limit=8)
+    for r, (topo, x0) in enumerate(zip(topos, x0s)):
+        m = ffi.Model(net, topo)
+        x = np.array(x0, dtype=np.uint32).copy()
+        st = ffi.run(m, x, wl["eps"], wl["nc"], SEED, np.inf, step_limit=8, realization=r)
+        assert np.array_equal(ctx.get_state(r), x), f"realization {r}: state differs"
+        _same_stats(ctx.stats(r), st, f"realization {r}")
     ctx.close()
 
 

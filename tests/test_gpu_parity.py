@@ -147,6 +147,33 @@ def test_traj_uneven_batch_ids():
     _traj_parity(wl["net"], wl["topos"], wl["x0"], t_end=0.05, steps=10**9, ids=ids)
 
 
+def _wide_network(S=10, R=20):
+    """S species, R first- and second-order channels X_a (+ X_b) -> X_c: more
+    than 8 species and 16 reactions, so the general instantiation runs and
+    the draws use the second coarse block of their counters (hops of species
+    8 and 9, reactions 16..19)."""
+    names = [f"X{s}" for s in range(S)]
+    reactions, rates = [], []
+    for j in range(R):
+        a, c = j % S, (j * 3 + 1) % S
+        if j % 4 == 3:
+            b = (j + 5) % S
+            reactions.append(({names[a]: 1, names[b]: 1}, {names[c]: 2}))
+            rates.append(2e-4 * (1 + j % 3))
+        else:
+            reactions.append(({names[a]: 1}, {names[c]: 1}))
+            rates.append(0.05 * (1 + j % 5))
+    return W.network(names, reactions, rates, [0.4 + 0.1 * (s % 4) for s in range(S)])
+
+
+def test_traj_wide_network_general_instantiation():
+    net = _wide_network()
+    topo = W.random_mesh(6, 6, 4, 3)
+    x0 = W.poisson_state(10, topo["N"], [300.0 + 40 * s for s in range(10)], 12)
+    _tau_parity(net, topo, x0)
+    _traj_parity(net, [topo], [x0], steps=40)
+
+
 def test_batch_equals_single():
     """A realization's trajectory does not depend on the batch it runs in."""
     wl = W.schlogl_batch(realizations=4, lo=4, hi=8)
