@@ -121,6 +121,7 @@ def gpu_measure(wl, steps, warmup, seed=2024, autotune=True, shape=None):
     leaps = sum(b["subvolume_leaps"] - a["subvolume_leaps"] for a, b in zip(s0, s1))
     fir = sum(b["firings"] - a["firings"] for a, b in zip(s0, s1))
     lbs = sum(b["leapbacks"] - a["leapbacks"] for a, b in zip(s0, s1))
+    crit = sum(b["crit_events"] - a["crit_events"] for a, b in zip(s0, s1))
     simt = sum((b["t"] - a["t"]) * n for a, b, n in zip(s0, s1, ctx.N))
     # per-kernel durations: HIP events around every launch on the context's stream
     # (the leap pass by kernel in separate iterations: the extra events
@@ -130,7 +131,7 @@ def gpu_measure(wl, steps, warmup, seed=2024, autotune=True, shape=None):
     prof += ctx.profile_kernels(n_prof)[5:]
     ctx.close()
     return {"seconds": dt, "accepted_steps": acc, "subvolume_leaps": leaps, "firings": fir,
-            "leapbacks": lbs, "subvolume_simtime": simt, "shape": shape, "tuned": (wpb, bpc),
+            "leapbacks": lbs, "crit_events": crit, "subvolume_simtime": simt, "shape": shape, "tuned": (wpb, bpc),
             "tau_ms": prof[1] / n_prof, "decide_ms": prof[2] / n_prof, "leap_ms": prof[3] / n_prof,
             "commit_ms": prof[4] / n_prof, "iter_ms_events": prof[0] / n_prof,
             "leap_kernel_ms": prof[5] / n_prof, "redo_ms": prof[6] / n_prof, "finish_ms": prof[7] / n_prof}
@@ -197,6 +198,7 @@ def learned_measure(wl, steps, warmup, train_steps, batch, seed=2024, kappa=1.0,
     return {"seconds": dt, "subvolume_leaps": d("subvolume_leaps"), "firings": d("firings"),
             "accepted_steps": d("steps"), "learned_steps": d("learned_steps"),
             "learned_rejects": d("learned_rejects"), "leapbacks": d("leapbacks"),
+            "crit_events": d("crit_events"),
             "rejection_rate": d("learned_rejects") / props if props else None,
             "subvolume_simtime": sum((b["t"] - a["t"]) * n for a, b, n in
                                      zip(s0, s1, [int(t["N"]) for t in wl["topos"]])),
@@ -425,6 +427,7 @@ def main():
         "firings_per_s": res["firings"] / res["seconds"],
         "accepted_steps": res["accepted_steps"],
         "leapbacks": res["leapbacks"],
+        "crit_events": res["crit_events"],  # steps that ended at tau'' (a critical event fired)
         "subvolume_simtime_per_s": res["subvolume_simtime"] / res["seconds"],
         "kernels": {"tau_ms_avg": res["tau_ms"], "leap_ms_avg": res["leap_ms"],
                     "decide_ms_avg": res["decide_ms"], "commit_ms_avg": res["commit_ms"],
