@@ -32,6 +32,8 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 PMC_TRAFFIC = os.path.join(ROOT, "profiles", "pmc_traffic.json")
 PMC_VALU = os.path.join(ROOT, "profiles", "pmc_valu.json")
+MLP_FLOP = 2 * (12 * 64 + 64 * 64 + 64 * 2)  # per subvolume and inference
+PEAK_BF16_TFLOPS = 2500.0  # MI355X dense bf16 MFMA
 
 
 def algorithmic_bytes(S, topos, part):
@@ -480,6 +482,18 @@ def main():
                         "subvolume_simtime_per_s": simt_learned,
                         "speedup_simulated_time": simt_learned / simt_plain if simt_plain else None,
                         **{k: v for k, v in lr_.items() if k not in ("seconds",)}}
+            # MFMA accounting: 2 (12x64 + 64x64 + 64x2) = 9,984 FLOP per subvolume per
+            # inference, training about 3x that per record
+            n_sub = int(sum(t["N"] for t in wl["topos"]))
+            inf_flop = MLP_FLOP * n_sub
+            mf = {"inference_flop_per_step": inf_flop, "peak_bf16_tflops": PEAK_BF16_TFLOPS,
+                  "inference_tflops_over_finish": inf_flop / (lr_["finish_with_proposals_ms"] * 1e-3) / 1e12}
+            mf["inference_frac_over_finish"] = mf["inference_tflops_over_finish"] / PEAK_BF16_TFLOPS
+            if lr_.get("train_steps") and lr_.get("train_seconds"):
+                tr = 3 * MLP_FLOP * lr_["train_batch"] * lr_["train_steps"] / lr_["train_seconds"] / 1e12
+                mf["train_tflops"] = tr
+                mf["train_frac"] = tr / PEAK_BF16_TFLOPS
+            out[key]["mfma"] = mf
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl, 2024, args.cpu_sample_steps)
     if not args.no_ref_benchmarks:
