@@ -84,6 +84,7 @@ struct plis_ctx {
     uint8_t* d_fown = nullptr;  // forward entry -> its row's index in the row's 64-subvolume tile
     unsigned long long* d_gdelta = nullptr;        // leap balances, counts layout
     float *d_w = nullptr, *d_iw = nullptr;
+    uint64_t *d_cw = nullptr, *d_icw = nullptr, *d_fcw = nullptr;  // interleaved id + weight
     int32_t* d_comp = nullptr;
     PartA* d_pa_t = nullptr;
     PartA* d_pa_c = nullptr;      // decide_kernel's chunk partials
@@ -323,6 +324,9 @@ static KernelArgs make_args(plis_ctx* c) {
     a.irp = c->d_irp;
     a.icol = c->d_icol;
     a.iw = c->d_iw;
+    a.cw = c->d_cw;
+    a.icw = c->d_icw;
+    a.fcw = c->d_fcw;
     a.comp = c->d_comp;
     a.partA_tile = c->d_pa_t;
     a.partA_chunk = c->d_pa_c;
@@ -580,6 +584,12 @@ extern "C" int plis_create(const plis_network* net, int32_t n_real, const plis_t
     std::vector<float> hfw;
     std::vector<uint8_t> hfown;
     std::vector<float> hw, hiw;
+    std::vector<uint64_t> hcw, hicw, hfcw;  // interleaved id + weight bits
+    auto idw = [](uint32_t u, float w) {
+        uint32_t b;
+        std::memcpy(&b, &w, 4);
+        return (uint64_t)u | ((uint64_t)b << 32);
+    };
     std::vector<double> hwsum;
     std::vector<int32_t> hcomp;
     std::vector<TileEnt> tiles;
@@ -621,6 +631,7 @@ extern "C" int plis_create(const plis_network* net, int32_t n_real, const plis_t
                 if (g.col[q] > v) {
                     hfcol.push_back(g.col[q]);
                     hfw.push_back(g.w[q]);
+                    hfcw.push_back(idw(g.col[q], g.w[q]));
                     hfown.push_back((uint8_t)(v % kTile));
                 }
         }
@@ -628,6 +639,7 @@ extern "C" int plis_create(const plis_network* net, int32_t n_real, const plis_t
         for (uint32_t v = 0; v < N; v += kTile)  // entry offsets of a tile's queue codes
             if (hfrp[rpoff + std::min(v + (uint32_t)kTile, N)] - hfrp[rpoff + v] >= kMaxTileEdges) huge_tile = true;
         hw.insert(hw.end(), g.w.begin(), g.w.end());
+        for (size_t q = 0; q < g.col.size(); ++q) hcw.push_back(idw(g.col[q], g.w[q]));
         // W_v = sum of out-edge weights, summed in CSR order in double
         for (uint32_t v = 0; v < N; ++v) {
             double W = 0.0;
@@ -644,6 +656,7 @@ extern "C" int plis_create(const plis_network* net, int32_t n_real, const plis_t
             hirp.insert(hirp.end(), in.rp.begin(), in.rp.end());
             hicol.insert(hicol.end(), in.col.begin(), in.col.end());
             hiw.insert(hiw.end(), in.w.begin(), in.w.end());
+            for (size_t q = 0; q < in.col.size(); ++q) hicw.push_back(idw(in.col[q], in.w[q]));
             ieoff += in.col.size();
             irpoff += N + 1;
         } else {
@@ -720,6 +733,7 @@ extern "C" int plis_create(const plis_network* net, int32_t n_real, const plis_t
         (rc = upload(c, &c->d_w, any_weighted ? hw : std::vector<float>())) ||
         (rc = upload(c, &c->d_irp, hirp)) || (rc = upload(c, &c->d_icol, hicol)) ||
         (rc = upload(c, &c->d_iw, hiw)) || (rc = upload(c, &c->d_comp, hcomp)) ||
+        (rc = upload(c, &c->d_cw, hcw)) || (rc = upload(c, &c->d_icw, hicw)) ||
         (rc = dev_alloc(c, &c->d_pa_t, c->ntiles)) || (rc = dev_alloc(c, &c->d_pb_t, c->ntiles)) ||
         (rc = dev_alloc(c, &c->d_redo, c->ntiles)) ||
         (rc = dev_alloc(c, &c->d_pa_c, (size_t)c->nreal * kDecideChunks)) ||
@@ -733,7 +747,7 @@ extern "C" int plis_create(const plis_network* net, int32_t n_real, const plis_t
     const bool ep = !any_asym && !huge_tile;
     if (ep && ((rc = upload(c, &c->d_frp, hfrp)) || (rc = upload(c, &c->d_fcol, hfcol)) ||
                       (rc = upload(c, &c->d_fw, any_weighted ? hfw : std::vector<float>())) ||
-                      (rc = upload(c, &c->d_fown, hfown)) ||
+                      (rc = upload(c, &c->d_fown, hfown)) || (rc = upload(c, &c->d_fcw, hfcw)) ||
                       (rc = upload(c, &c->d_gdelta, std::vector<unsigned long long>(hx.size(), 0ull))))) {
         g_create_err = c->err;
         plis_destroy(c);

@@ -110,6 +110,9 @@ __device__ __forceinline__ uint32_t tile_at(uint32_t it, uint32_t ntiles, uint32
     return t < ce ? t : 0xFFFFFFFFu;
 }
 
+// an interleaved edge entry's high word for weight 1.0 (a padding entry)
+constexpr uint64_t kOneW = (uint64_t)0x3F800000u << 32;
+
 __device__ __forceinline__ double binom_d(uint32_t x, int r) {
     if (x < (uint32_t)r) return 0.0;  // r >= 1
     const double xd = (double)x;
@@ -459,10 +462,9 @@ __device__ __forceinline__ void tau_subvolume(const KernelArgs& a, const NetView
     const int c = n.C > 1 ? a.comp[ri.comp_off + v] : 0;
     TauAcc<MS> t;
     tau_own<MS, MR>(a, n, ri, X, v, c, t);
-    const bool weighted = (ri.flags & 2u) != 0;
     const bool sym = (ri.flags & 1u) != 0;
-    const uint32_t* icol = sym ? a.col + ri.e_off : a.icol + ri.ie_off;
-    const float* iwp = sym ? a.w + ri.e_off : a.iw + ri.ie_off;
+    // in-edges: neighbour id and weight in one 8-byte load
+    const uint64_t* icw = sym ? a.cw + ri.e_off : a.icw + ri.ie_off;
     const uint32_t* irp = sym ? a.rp + ri.rp_off : a.irp + ri.irp_off;
     const uint32_t ib = irp[v], ie = irp[v + 1];
     // one compartment: the neighbours' diffusion constants are the tile's
@@ -472,10 +474,12 @@ __device__ __forceinline__ void tau_subvolume(const KernelArgs& a, const NetView
     // software pipeline: in-edge q's counts were requested in iteration q-1,
     // in-edge q+1's id and weight in iteration q-1 too, so each iteration
     // waits on nothing issued in it
-    uint32_t u_cur = ib < ie ? icol[ib] : v;
-    float w_cur = (ib < ie && weighted) ? iwp[ib] : 1.0f;
-    uint32_t u_next = ib + 1u < ie ? icol[ib + 1u] : v;
-    float w_next = (ib + 1u < ie && weighted) ? iwp[ib + 1u] : 1.0f;
+    const uint64_t cw0 = ib < ie ? icw[ib] : (uint64_t)v | kOneW;
+    const uint64_t cw1 = ib + 1u < ie ? icw[ib + 1u] : (uint64_t)v | kOneW;
+    uint32_t u_cur = (uint32_t)cw0;
+    float w_cur = __uint_as_float((uint32_t)(cw0 >> 32));
+    uint32_t u_next = (uint32_t)cw1;
+    float w_next = __uint_as_float((uint32_t)(cw1 >> 32));
     uint32_t xu[MS];
 #pragma unroll
     for (int s = 0; s < MS; ++s) xu[s] = X[ri.x_off + (uint64_t)(s < n.S ? s : n.S - 1) * N + u_cur];
@@ -491,8 +495,9 @@ __device__ __forceinline__ void tau_subvolume(const KernelArgs& a, const NetView
 #pragma unroll
             for (int s = 0; s < MS; ++s) xu[s] = X[ri.x_off + (uint64_t)(s < n.S ? s : n.S - 1) * N + u_cur];
             if (q + 2u < ie) {
-                u_next = icol[q + 2u];
-                w_next = weighted ? iwp[q + 2u] : 1.0f;
+                const uint64_t cw2 = icw[q + 2u];
+                u_next = (uint32_t)cw2;
+                w_next = __uint_as_float((uint32_t)(cw2 >> 32));
             }
         }
         if (n.C > 1) {
@@ -981,9 +986,7 @@ __device__ __forceinline__ uint32_t walk_fwd_draws(const KernelArgs& a, const Ne
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t N = ri.N;
     const int nc = a.n_critical;
-    const bool weighted = (ri.flags & 2u) != 0;
-    const uint32_t* colp = a.fcol + ri.fe_off;
-    const float* wp = a.fw + ri.fe_off;
+    const uint64_t* cwp = a.fcw + ri.fe_off;  // neighbour id and weight in one load
     const double tau = sv.tau;
     const float tauf = (float)tau;
     const uint32_t k0 = a.seed_lo, k1 = a.seed_hi;
@@ -1011,9 +1014,11 @@ __device__ __forceinline__ uint32_t walk_fwd_draws(const KernelArgs& a, const Ne
     };
     const uint8_t* ownp = a.fown + ri.fe_off;
     const uint32_t e0 = eb + lane;
-    uint32_t u_cur = e0 < ee ? colp[e0] : v0, u_next = e0 + (uint32_t)kTile < ee ? colp[e0 + kTile] : v0;
-    float w_cur = (e0 < ee && weighted) ? wp[e0] : 1.0f;
-    float w_next = (e0 + (uint32_t)kTile < ee && weighted) ? wp[e0 + kTile] : 1.0f;
+    const uint64_t cw0 = e0 < ee ? cwp[e0] : (uint64_t)v0 | kOneW;
+    const uint64_t cw1 = e0 + (uint32_t)kTile < ee ? cwp[e0 + kTile] : (uint64_t)v0 | kOneW;
+    uint32_t u_cur = (uint32_t)cw0, u_next = (uint32_t)cw1;
+    float w_cur = __uint_as_float((uint32_t)(cw0 >> 32));
+    float w_next = __uint_as_float((uint32_t)(cw1 >> 32));
     uint32_t i_cur = e0 < ee ? ownp[e0] : 0u, i_next = e0 + (uint32_t)kTile < ee ? ownp[e0 + kTile] : 0u;
     uint32_t xn[MS];
 #pragma unroll
@@ -1036,8 +1041,9 @@ __device__ __forceinline__ uint32_t walk_fwd_draws(const KernelArgs& a, const Ne
         if (e + (uint32_t)kTile < ee) {
             counts(u_cur, xn);
             if (e + 2u * (uint32_t)kTile < ee) {
-                u_next = colp[e + 2u * kTile];
-                w_next = weighted ? wp[e + 2u * kTile] : 1.0f;
+                const uint64_t cw2 = cwp[e + 2u * kTile];
+                u_next = (uint32_t)cw2;
+                w_next = __uint_as_float((uint32_t)(cw2 >> 32));
                 i_next = ownp[e + 2u * kTile];
             }
         }

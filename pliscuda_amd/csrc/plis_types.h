@@ -101,6 +101,12 @@ struct KernelArgs {
     const uint32_t* irp;
     const uint32_t* icol;
     const float* iw;
+    // the same edges interleaved, one 8-byte load per edge: neighbour id in
+    // the low word, weight's float bits in the high word (1.0 unweighted) --
+    // cw parallel to col / w, icw to icol / iw, fcw to fcol / fw
+    const uint64_t* cw;
+    const uint64_t* icw;
+    const uint64_t* fcw;
     const int32_t* comp;
     PartA* partA_tile;
     PartA* partA_chunk;      // decide_kernel: one partial per (realization, chunk of its tiles)
