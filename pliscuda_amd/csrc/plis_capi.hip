@@ -79,12 +79,11 @@ struct plis_ctx {
     uint32_t *d_x0 = nullptr, *d_x1 = nullptr;
     uint32_t *d_rp = nullptr, *d_col = nullptr, *d_irp = nullptr, *d_icol = nullptr;
     double* d_wsum = nullptr;  // per subvolume, row_ptr layout
-    uint32_t *d_frp = nullptr, *d_fcol = nullptr;  // forward CSR (symmetric realizations)
-    float* d_fw = nullptr;
-    uint8_t* d_fown = nullptr;  // forward entry -> its row's index in the row's 64-subvolume tile
+    uint32_t* d_frp = nullptr;  // forward CSR (symmetric realizations): row pointers
+    uint64_t* d_fcw = nullptr;  // its entries: neighbour id, row's index in its tile, weight
     unsigned long long* d_gdelta = nullptr;        // leap balances, counts layout
     float *d_w = nullptr, *d_iw = nullptr;
-    uint64_t *d_cw = nullptr, *d_icw = nullptr, *d_fcw = nullptr;  // interleaved id + weight
+    uint64_t *d_cw = nullptr, *d_icw = nullptr;  // interleaved id + weight
     int32_t* d_comp = nullptr;
     PartA* d_pa_t = nullptr;
     PartA* d_pa_c = nullptr;      // decide_kernel's chunk partials
@@ -337,9 +336,7 @@ static KernelArgs make_args(plis_ctx* c) {
     a.redo_count = c->d_cnt + 1;
     a.redo_tiles = c->d_redo;
     a.frp = c->d_frp;
-    a.fcol = c->d_fcol;
-    a.fw = c->d_fw;
-    a.fown = c->d_fown;
+
     a.gdelta = c->d_gdelta;
     a.redo_total = reinterpret_cast<unsigned long long*>(c->d_cnt + 2);
     a.epsilon = c->eps;
@@ -580,9 +577,7 @@ extern "C" int plis_create(const plis_network* net, int32_t n_real, const plis_t
     c->nreal = (uint32_t)n_real;
 
     // host-side layout
-    std::vector<uint32_t> hx, hrp, hcol, hirp, hicol, hfrp, hfcol;
-    std::vector<float> hfw;
-    std::vector<uint8_t> hfown;
+    std::vector<uint32_t> hx, hrp, hcol, hirp, hicol, hfrp;
     std::vector<float> hw, hiw;
     std::vector<uint64_t> hcw, hicw, hfcw;  // interleaved id + weight bits
     auto idw = [](uint32_t u, float w) {
@@ -624,18 +619,14 @@ extern "C" int plis_create(const plis_network* net, int32_t n_real, const plis_t
         hrp.insert(hrp.end(), g.rp.begin(), g.rp.end());
         hcol.insert(hcol.end(), g.col.begin(), g.col.end());
         // forward CSR: each undirected edge once, in the row of its lower end
-        ri.fe_off = hfcol.size();
+        ri.fe_off = hfcw.size();
         for (uint32_t v = 0; v < N; ++v) {
-            hfrp.push_back((uint32_t)(hfcol.size() - ri.fe_off));
+            hfrp.push_back((uint32_t)(hfcw.size() - ri.fe_off));
             for (uint32_t q = g.rp[v]; q < g.rp[v + 1]; ++q)
-                if (g.col[q] > v) {
-                    hfcol.push_back(g.col[q]);
-                    hfw.push_back(g.w[q]);
-                    hfcw.push_back(idw(g.col[q], g.w[q]));
-                    hfown.push_back((uint8_t)(v % kTile));
-                }
+                if (g.col[q] > v) hfcw.push_back(idw((g.col[q] & kFwdIdMask) | ((v % kTile) << 26), g.w[q]));
         }
-        hfrp.push_back((uint32_t)(hfcol.size() - ri.fe_off));
+        hfrp.push_back((uint32_t)(hfcw.size() - ri.fe_off));
+        if (N > kFwdIdMask) huge_tile = true;  // ids take 26 bits of a forward entry
         for (uint32_t v = 0; v < N; v += kTile)  // entry offsets of a tile's queue codes
             if (hfrp[rpoff + std::min(v + (uint32_t)kTile, N)] - hfrp[rpoff + v] >= kMaxTileEdges) huge_tile = true;
         hw.insert(hw.end(), g.w.begin(), g.w.end());
@@ -745,9 +736,7 @@ extern "C" int plis_create(const plis_network* net, int32_t n_real, const plis_t
     }
     // the forward CSR and the zeroed balances serve the edge-parallel leap
     const bool ep = !any_asym && !huge_tile;
-    if (ep && ((rc = upload(c, &c->d_frp, hfrp)) || (rc = upload(c, &c->d_fcol, hfcol)) ||
-                      (rc = upload(c, &c->d_fw, any_weighted ? hfw : std::vector<float>())) ||
-                      (rc = upload(c, &c->d_fown, hfown)) || (rc = upload(c, &c->d_fcw, hfcw)) ||
+    if (ep && ((rc = upload(c, &c->d_frp, hfrp)) || (rc = upload(c, &c->d_fcw, hfcw)) ||
                       (rc = upload(c, &c->d_gdelta, std::vector<unsigned long long>(hx.size(), 0ull))))) {
         g_create_err = c->err;
         plis_destroy(c);

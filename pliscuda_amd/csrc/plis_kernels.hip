@@ -986,7 +986,7 @@ __device__ __forceinline__ uint32_t walk_fwd_draws(const KernelArgs& a, const Ne
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t N = ri.N;
     const int nc = a.n_critical;
-    const uint64_t* cwp = a.fcw + ri.fe_off;  // neighbour id and weight in one load
+    const uint64_t* cwp = a.fcw + ri.fe_off;  // neighbour, row in the tile and weight in one load
     const double tau = sv.tau;
     const float tauf = (float)tau;
     const uint32_t k0 = a.seed_lo, k1 = a.seed_hi;
@@ -1012,14 +1012,13 @@ __device__ __forceinline__ uint32_t walk_fwd_draws(const KernelArgs& a, const Ne
 #pragma unroll
         for (int s = 0; s < MS; ++s) xo[s] = X[ri.x_off + (uint64_t)(s < n.S ? s : n.S - 1) * N + u];
     };
-    const uint8_t* ownp = a.fown + ri.fe_off;
     const uint32_t e0 = eb + lane;
     const uint64_t cw0 = e0 < ee ? cwp[e0] : (uint64_t)v0 | kOneW;
     const uint64_t cw1 = e0 + (uint32_t)kTile < ee ? cwp[e0 + kTile] : (uint64_t)v0 | kOneW;
-    uint32_t u_cur = (uint32_t)cw0, u_next = (uint32_t)cw1;
+    uint32_t u_cur = (uint32_t)cw0 & kFwdIdMask, u_next = (uint32_t)cw1 & kFwdIdMask;
+    uint32_t i_cur = (uint32_t)cw0 >> 26, i_next = (uint32_t)cw1 >> 26;
     float w_cur = __uint_as_float((uint32_t)(cw0 >> 32));
     float w_next = __uint_as_float((uint32_t)(cw1 >> 32));
-    uint32_t i_cur = e0 < ee ? ownp[e0] : 0u, i_next = e0 + (uint32_t)kTile < ee ? ownp[e0 + kTile] : 0u;
     uint32_t xn[MS];
 #pragma unroll
     for (int s = 0; s < MS; ++s) xn[s] = 0u;
@@ -1042,9 +1041,9 @@ __device__ __forceinline__ uint32_t walk_fwd_draws(const KernelArgs& a, const Ne
             counts(u_cur, xn);
             if (e + 2u * (uint32_t)kTile < ee) {
                 const uint64_t cw2 = cwp[e + 2u * kTile];
-                u_next = (uint32_t)cw2;
+                u_next = (uint32_t)cw2 & kFwdIdMask;
+                i_next = (uint32_t)cw2 >> 26;
                 w_next = __uint_as_float((uint32_t)(cw2 >> 32));
-                i_next = ownp[e + 2u * kTile];
             }
         }
         const bool local = u - v0 < (uint32_t)kTile;
@@ -1166,11 +1165,12 @@ __device__ __forceinline__ QueuedHop queued_hop(const KernelArgs& a, const NetVi
     QueuedHop h;
     const uint32_t e = eb + (code >> 4);
     h.w = code & 15u;
-    h.i = a.fown[ri.fe_off + e];
-    h.u = a.fcol[ri.fe_off + e];
+    const uint64_t cw = a.fcw[ri.fe_off + e];
+    h.i = (uint32_t)cw >> 26;
+    h.u = (uint32_t)cw & kFwdIdMask;
     h.s = (int)(h.w >> 1);
     h.fwd = (h.w & 1u) == 0u;
-    const float wf = (ri.flags & 2u) ? a.fw[ri.fe_off + e] : 1.0f;
+    const float wf = __uint_as_float((uint32_t)(cw >> 32));
     const double wq = (double)wf;
     double d;
     uint32_t xs;

@@ -16,6 +16,7 @@
 namespace plis {
 
 constexpr int kTile = 64;        // subvolumes per tile (one wavefront)
+constexpr uint32_t kFwdIdMask = (1u << 26) - 1u;  // neighbour id bits of a forward entry
 constexpr int kMaxS = 64;
 constexpr int kMaxR = 128;
 
@@ -103,10 +104,9 @@ struct KernelArgs {
     const float* iw;
     // the same edges interleaved, one 8-byte load per edge: neighbour id in
     // the low word, weight's float bits in the high word (1.0 unweighted) --
-    // cw parallel to col / w, icw to icol / iw, fcw to fcol / fw
+    // cw parallel to col / w, icw to icol / iw
     const uint64_t* cw;
     const uint64_t* icw;
-    const uint64_t* fcw;
     const int32_t* comp;
     PartA* partA_tile;
     PartA* partA_chunk;      // decide_kernel: one partial per (realization, chunk of its tiles)
@@ -116,12 +116,14 @@ struct KernelArgs {
     uint32_t* ndone;
     uint32_t* redo_count;    // tiles phase B left to the redo pass (reset by decide)
     uint32_t* redo_tiles;
-    // forward CSR (row_ptr layout of rp; entries u > v) and the leap pass's
-    // global per-subvolume balances (counts layout, zero between passes)
+    // forward CSR (row_ptr layout of rp; entries v -> u with u > v) and the
+    // leap pass's global per-subvolume balances (counts layout, zero between
+    // passes).  An entry is one 8-byte word: u in bits 0..25, the index of
+    // its row v within v's tile in bits 26..31, the weight's float bits in
+    // the high word (1.0 unweighted); realizations of the edge-parallel
+    // pass have fewer than 2^26 subvolumes.
     const uint32_t* frp;
-    const uint32_t* fcol;
-    const float* fw;
-    const uint8_t* fown;  // per forward entry: its row's index within the row's tile
+    const uint64_t* fcw;
     unsigned long long* gdelta;
     unsigned long long* redo_total;  // tiles redone since the context was made
     double epsilon;
