@@ -87,6 +87,7 @@ struct plis_ctx {
     int32_t* d_comp = nullptr;
     PartA* d_pa_t = nullptr;
     PartA* d_pa_c = nullptr;      // decide_kernel's chunk partials
+    PartB* d_pb_c = nullptr;      // commit_decide_kernel's chunk partials of the leap sums
     uint32_t decide_chunks = 1;   // decide_kernel's blocks per realization
     uint32_t* d_dcnt = nullptr;   // decide_kernel's per-realization chunk counters
     PartB* d_pb_t = nullptr;
@@ -329,6 +330,7 @@ static KernelArgs make_args(plis_ctx* c) {
     a.comp = c->d_comp;
     a.partA_tile = c->d_pa_t;
     a.partA_chunk = c->d_pa_c;
+    a.partB_chunk = c->d_pb_c;
     a.decide_cnt = c->d_dcnt;
     a.decide_chunks = c->decide_chunks;
     a.partB_tile = c->d_pb_t;
@@ -430,6 +432,16 @@ static void launch_variant(plis_ctx* c, int which, const KernelArgs& a) {
         case 3:
             hipLaunchKernelGGL(commit_kernel, rgrid, rblock, 0, c->stream, a);
             break;
+        case 4: {  // folded order: the tau pass on the counts the leap pass wrote
+            KernelArgs as = a;
+            as.tau_spec = 1u;
+            if (mc) hipLaunchKernelGGL((phase_a_kernel<MS, MR, true>), grid, block, 0, c->stream, as);
+            else hipLaunchKernelGGL((phase_a_kernel<MS, MR, false>), grid, block, 0, c->stream, as);
+            break;
+        }
+        case 6:  // folded order: commit and decide
+            hipLaunchKernelGGL((commit_decide_kernel<MS, MR>), dgrid, dblock, 0, c->stream, a);
+            break;
         case 5:
             if constexpr (MS <= 8) hipLaunchKernelGGL((records_kernel<MS, MR>), grid, block, 0, c->stream, a);
             break;
@@ -448,8 +460,30 @@ static void launch_kernel(plis_ctx* c, int which, const KernelArgs& a) {
     }
 }
 
+// The kernels of one iteration.  Plain tau (the folded order): the leap
+// pass, the tau pass on the counts it wrote (used if the step is accepted),
+// then one kernel that commits the step or leaps back and decides the next
+// step.  A learned context: tau pass, decide, leap pass, commit.
+static bool folded(const plis_ctx* c) { return !c->learned_on; }
+
+static const int kFoldedStep[3] = {2, 4, 6};
+
 static void launch_step(plis_ctx* c, const KernelArgs& a) {
-    for (int k = 0; k < 4; ++k) launch_kernel(c, k, a);
+    if (folded(c))
+        for (int k : kFoldedStep) launch_kernel(c, k, a);
+    else
+        for (int k = 0; k < 4; ++k) launch_kernel(c, k, a);
+}
+
+// Before the first iteration of a call in the folded order: the tau pass on
+// the current counts and the decision, which each folded iteration makes
+// for the next one at its end.  Recomputing a decision that is already in
+// place gives the same values (every input is the same), so a call may
+// always start with it.
+static void launch_prologue(plis_ctx* c, const KernelArgs& a) {
+    if (!folded(c)) return;
+    launch_kernel(c, 0, a);
+    launch_kernel(c, 1, a);
 }
 
 // The captured graphs hold the kernel arguments and launch shape by value:
@@ -474,6 +508,12 @@ static void set_shape(plis_ctx* c, uint32_t wpb, uint32_t bpc) {
 static int launch_iters(plis_ctx* c, uint32_t n) {
     const KernelArgs a = make_args(c);
     for (uint32_t i = 0; i < n; ++i) launch_step(c, a);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
+static int prologue(plis_ctx* c) {
+    launch_prologue(c, make_args(c));
     HIPCHK(c, hipGetLastError());
     return 0;
 }
@@ -728,6 +768,7 @@ extern "C" int plis_create(const plis_network* net, int32_t n_real, const plis_t
         (rc = dev_alloc(c, &c->d_pa_t, c->ntiles)) || (rc = dev_alloc(c, &c->d_pb_t, c->ntiles)) ||
         (rc = dev_alloc(c, &c->d_redo, c->ntiles)) ||
         (rc = dev_alloc(c, &c->d_pa_c, (size_t)c->nreal * kDecideChunks)) ||
+        (rc = dev_alloc(c, &c->d_pb_c, (size_t)c->nreal * kDecideChunks)) ||
         (rc = upload(c, &c->d_dcnt, std::vector<uint32_t>(c->nreal, 0u))) ||
         (rc = upload(c, &c->d_cnt, zeros_cnt))) {
         g_create_err = c->err;
@@ -816,6 +857,8 @@ extern "C" int plis_run_until(plis_ctx* c, double t_end, double t_pause, uint64_
         return p;
     };
     last_progress = progress();
+    if (ndone < c->nreal)
+        if (int rc = prologue(c)) return rc;
     while (ndone < c->nreal) {
         if (int rc = graph_iters(c, k)) return rc;
         HIPCHK(c, hipMemcpyAsync(c->h_ndone, d_ndone, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
@@ -850,6 +893,8 @@ extern "C" int plis_run(plis_ctx* c, double t_end, uint64_t step_limit) {
 
 extern "C" int plis_iterate(plis_ctx* c, uint32_t n) {
     if (!c) return PLIS_ERR_ARG;
+    if (n > 0)
+        if (int rc = prologue(c)) return rc;
     if (int rc = graph_iters(c, n)) return rc;
     return PLIS_OK;
 }
@@ -858,16 +903,25 @@ extern "C" int plis_profile_kernels(plis_ctx* c, uint32_t n, double* out) {
     if (!c || !out) return PLIS_ERR_ARG;
     // per iteration: before/after each of the 4 phases, and 2 events
     // between the leap pass's kernels
+    // (the folded order launches the leap pass, the tau pass and commit +
+    // decide; its decide slot stays empty and its commit slot times the
+    // merged kernel)
     std::vector<hipEvent_t> ev(10 * (size_t)n + 2);
     for (auto& e : ev) HIPCHK(c, hipEventCreate(&e));
     const KernelArgs a = make_args(c);
+    if (n > 0)
+        if (int rc = prologue(c)) return rc;
+    const bool fold = folded(c);
     HIPCHK(c, hipEventRecord(ev[0], c->stream));
     for (uint32_t i = 0; i < n; ++i)
-        for (int k = 0; k < 4; ++k) {
+        for (int j = 0; j < 4; ++j) {
+            // slot k of the output, kernel group `which`, in launch order
+            const int k = fold ? (j == 0 ? 2 : j == 1 ? 0 : j == 2 ? 1 : 3) : j;
+            const int which = fold ? (k == 0 ? 4 : k == 3 ? 6 : k) : k;
             hipEvent_t* e = &ev[1 + 10 * (size_t)i];
             HIPCHK(c, hipEventRecord(e[2 * k], c->stream));
             c->split_ev = k == 2 ? &e[8] : nullptr;
-            launch_kernel(c, k, a);
+            if (!(fold && k == 1)) launch_kernel(c, which, a);
             c->split_ev = nullptr;
             HIPCHK(c, hipEventRecord(e[2 * k + 1], c->stream));
         }
@@ -1110,14 +1164,16 @@ extern "C" int plis_launch_shape(plis_ctx* c, int32_t* out) {
 constexpr uint32_t kPoisonCount = 0x7FFFFFF0u;
 
 __global__ __launch_bounds__(256) void poison_parts_kernel(PartA* pa, PartB* pb, uint32_t* redo, uint32_t ntiles,
-                                                           PartA* pc, uint32_t nchunks) {
+                                                           PartA* pc, PartB* pbc, uint32_t nchunks) {
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < ntiles; t += gridDim.x * blockDim.x) {
         pa[t] = PartA{1e-300, 1e-300, 0u, 0xDEADu};
         pb[t] = PartB{1ull << 40, 1u, 7u};
         redo[t] = 0u;
     }
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nchunks; t += gridDim.x * blockDim.x)
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nchunks; t += gridDim.x * blockDim.x) {
         pc[t] = PartA{1e-300, 1e-300, 0u, 0xDEADu};
+        pbc[t] = PartB{1ull << 40, 1u, 7u};
+    }
 }
 
 extern "C" int plis_debug_poison(plis_ctx* c) {
@@ -1129,7 +1185,7 @@ extern "C" int plis_debug_poison(plis_ctx* c) {
         HIPCHK(c, hipMemsetD32Async(other + ri.x_off, (int)kPoisonCount, (size_t)c->S * ri.N, c->stream));
     }
     hipLaunchKernelGGL(poison_parts_kernel, dim3(std::min<uint32_t>((c->ntiles + 255) / 256, 1024u)), dim3(256), 0,
-                       c->stream, c->d_pa_t, c->d_pb_t, c->d_redo, c->ntiles, c->d_pa_c,
+                       c->stream, c->d_pa_t, c->d_pb_t, c->d_redo, c->ntiles, c->d_pa_c, c->d_pb_c,
                        c->nreal * kDecideChunks);
     HIPCHK(c, hipGetLastError());
     // the partials no longer hold any proposal

@@ -554,11 +554,25 @@ __device__ void finalize_A(const KernelArgs& a, const NetView& n, const RealInfo
 
 // Cao's tau' / tau'' of one tile's subvolumes into the tile partial (the
 // debug outputs, when given, get every subvolume's values instead).
+// (tp, t2, v2) <- min tau', argmin tau'' (lowest subvolume on ties) of
+// itself and (otp, ot2, ov)
+__device__ __forceinline__ void merge_partA(double& tp, double& t2, uint32_t& v2, double otp, double ot2,
+                                            uint32_t ov) {
+    tp = fmin(tp, otp);
+    if (ot2 < t2 || (ot2 == t2 && ov < v2)) {
+        t2 = ot2;
+        v2 = ov;
+    }
+}
+
+// spec: the counts the leap pass just wrote (the buffer that becomes current
+// if the step is accepted) and the next step's counter.
 template <int MS, int MR>
 __device__ __forceinline__ void cao_tau_tile(const KernelArgs& a, const NetView& n, const RealInfo& ri,
-                                             const cptr<RealCtl> cp, const TileEnt& te, uint32_t t, bool dbg) {
+                                             const cptr<RealCtl> cp, const TileEnt& te, uint32_t t, bool dbg,
+                                             bool spec = false) {
     const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t cur = wave_uniform(cp->cur);
+    const uint32_t cur = wave_uniform(cp->cur) ^ (spec ? 1u : 0u);
     const uint32_t* X = cur ? a.x1 : a.x0;
     const uint32_t v = te.v0 + lane;
     double tp = kInf, t2 = kInf, a0c = 0.0;
@@ -566,7 +580,7 @@ __device__ __forceinline__ void cao_tau_tile(const KernelArgs& a, const NetView&
     if (v < ri.N) {
         tau_subvolume<MS, MR>(a, n, ri, X, v, tp, a0c);
         if (a0c > 0.0) {
-            const uint32_t step = wave_uniform(cp->step);
+            const uint32_t step = wave_uniform(cp->step) + (spec ? 1u : 0u);
             const U4 o = philox10(v, kSelfId, step, ctr_word3(ri.id, 0u, kTau2, 0u), a.seed_lo, a.seed_hi);
             t2 = -dlog(uniform01(o.x)) / a0c;
         }
@@ -604,8 +618,11 @@ void phase_a_kernel(KernelArgs a) {
         const cptr<RealCtl> cp = ro(a.ctl + r);
         const uint32_t done = wave_uniform(cp->done), retry = wave_uniform(cp->retry);
         const bool dbg = a.dbg_tp != nullptr;
-        if (!dbg && (done || retry)) continue;
-        cao_tau_tile<MS, MR>(a, n, ri, cp, te, t, dbg);
+        // (in the folded order a retried leap has just been made, so only
+        // realizations that did not leap are skipped)
+        const bool spec = a.tau_spec != 0u && !dbg;
+        if (!dbg && (done || (retry && !spec))) continue;
+        cao_tau_tile<MS, MR>(a, n, ri, cp, te, t, dbg, spec);
     }
 }
 
@@ -1350,8 +1367,11 @@ __device__ __forceinline__ void reduce_partB(unsigned long long& fire, uint32_t&
     }
 }
 
-__device__ void finalize_B(const KernelArgs& a, const RealInfo& ri, uint32_t r, unsigned long long fire,
-                           uint32_t bad, uint32_t viol, RealCtl c) {
+// Commits the step or leaps back, on c: returns true when the step was
+// accepted and the realization runs on (its next step is then to be
+// decided).  lead: the one thread that counts a finished realization in.
+__device__ bool commit_step(const KernelArgs& a, const RealInfo& ri, unsigned long long fire, uint32_t bad,
+                            uint32_t viol, RealCtl& c, bool lead) {
     // A learned proposal is judged by Cao's leap condition after the fact: it
     // is kept only without negative populations and with no more violating
     // subvolumes than the plain method produces -- mean mu = viol_rate_max N,
@@ -1372,9 +1392,8 @@ __device__ void finalize_B(const KernelArgs& a, const RealInfo& ri, uint32_t r, 
         if (c.attempt > kMaxAttempt) {
             c.status = kErrLeapback;
             c.done = 1u;
-            a.ctl[r] = c;
-            atomicAdd(a.ndone, 1u);
-            return;
+            if (lead) atomicAdd(a.ndone, 1u);
+            return false;
         }
         if (!learned_reject) {
             c.tau_p = c.tau_p * 0.5;
@@ -1382,8 +1401,7 @@ __device__ void finalize_B(const KernelArgs& a, const RealInfo& ri, uint32_t r, 
             c.retry = 1u;
         }
         c.prop_ok = 0u;
-        a.ctl[r] = c;
-        return;
+        return false;
     }
     // the leap pass of a learned context proposed the next step's tau from
     // the state it produced (edge-parallel pass only)
@@ -1403,8 +1421,15 @@ __device__ void finalize_B(const KernelArgs& a, const RealInfo& ri, uint32_t r, 
     if (c.t >= c.t_end || c.steps >= c.step_limit || c.t >= c.t_pause) {
         c.done = 1u;
         if (c.t >= c.t_end || c.steps >= c.step_limit) c.status = kDone;
-        atomicAdd(a.ndone, 1u);
+        if (lead) atomicAdd(a.ndone, 1u);
+        return false;
     }
+    return true;
+}
+
+__device__ void finalize_B(const KernelArgs& a, const RealInfo& ri, uint32_t r, unsigned long long fire,
+                           uint32_t bad, uint32_t viol, RealCtl c) {
+    commit_step(a, ri, fire, bad, viol, c, true);
     a.ctl[r] = c;
 }
 
@@ -1574,6 +1599,106 @@ __global__ __launch_bounds__(1024) void commit_kernel(KernelArgs a) {
     }
 }
 
+// The folded order of a plain-tau iteration: commits the step or leaps
+// back from the leap pass's tile partials and, when the step is accepted
+// and the realization runs on, decides the next step from the tile
+// partials of the tau pass that ran on the new counts (phase_a_kernel with
+// tau_spec) -- commit_kernel and decide_kernel in one launch.  Blocks and
+// chunks as in decide_kernel (decide_chunks blocks per realization, the
+// last one finishes; counter shared with decide_kernel, which never runs
+// at the same time).  A leap-back drops the tau pass's partials (the
+// retried leap reuses the halved tau).
+template <int MS, int MR>
+__global__ __launch_bounds__(decide_threads<MS>()) void commit_decide_kernel(KernelArgs a) {
+    const uint32_t nch = a.decide_chunks;
+    const uint32_t r = blockIdx.x / nch, chunk = blockIdx.x % nch;
+    if (blockIdx.x == 0u && threadIdx.x == 0u) *a.redo_count = 0u;  // the coming leap pass's redo list
+    if (r >= a.nreal) return;
+    const RealInfo ri = ld_ro(a.info + r);
+    const RealCtl c0 = ld_ro(a.ctl + r);
+    __shared__ unsigned long long s_f[16];
+    __shared__ uint32_t s_b[16], s_v[16];
+    __shared__ double s_tp[16], s_t2[16];
+    __shared__ uint32_t s_v2[16];
+    __shared__ uint32_t s_last;
+    unsigned long long fire = 0ull;
+    uint32_t bad = 0u, viol = 0u;
+    double tp = kInf, t2 = kInf;
+    uint32_t v2 = 0xFFFFFFFFu;
+    const uint32_t per = (ri.ntiles + nch - 1u) / nch;
+    const uint32_t qb = min(chunk * per, ri.ntiles), qe = min(qb + per, ri.ntiles);
+    const PartB* pb = a.partB_tile + ri.tile_begin;
+    const PartA* pa = a.partA_tile + ri.tile_begin;
+    // both partials' loads in flight together
+    for (uint32_t q0 = qb; q0 < qe; q0 += 8u * blockDim.x) {
+        PartB b[8];
+        PartA p[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const uint32_t q = q0 + (uint32_t)i * blockDim.x + threadIdx.x;
+            b[i] = q < qe ? pb[q] : PartB{0ull, 0u, 0u};
+            p[i] = q < qe ? pa[q] : PartA{kInf, kInf, 0xFFFFFFFFu, 0u};
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            fire += b[i].fire;
+            bad |= b[i].bad;
+            viol += b[i].viol;
+            merge_partA(tp, t2, v2, p[i].tp, p[i].t2, p[i].v2);
+        }
+    }
+    if (c0.done) return;  // (the same in every block of the realization)
+    reduce_partB(fire, bad, viol);
+    const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    if (lane == 0) {
+        s_f[wv] = fire;
+        s_b[wv] = bad;
+        s_v[wv] = viol;
+    }
+    reduce_block_partA<MS>(tp, t2, v2, s_tp, s_t2, s_v2);  // (its barrier publishes s_f / s_b / s_v)
+    if (wv == 0)  // the first wavefront, every lane alike
+        for (uint32_t w = 1; w < (blockDim.x >> 6); ++w) {
+            fire += s_f[w];
+            bad |= s_b[w];
+            viol += s_v[w];
+        }
+    if (nch > 1u) {
+        if (threadIdx.x == 0) {
+            a.partA_chunk[r * nch + chunk] = PartA{tp, t2, v2, 0u};
+            a.partB_chunk[r * nch + chunk] = PartB{fire, bad, viol};
+            __threadfence();
+            s_last = atomicAdd(&a.decide_cnt[r], 1u) == nch - 1u ? 1u : 0u;
+        }
+        __syncthreads();
+        if (!s_last || wv != 0) return;
+        // the last block of the realization: its first wavefront finishes
+        __threadfence();
+        PartA p{kInf, kInf, 0xFFFFFFFFu, 0u};
+        PartB b{0ull, 0u, 0u};
+        if (lane < nch) {
+            p = a.partA_chunk[r * nch + lane];
+            b = a.partB_chunk[r * nch + lane];
+        }
+        tp = p.tp;
+        t2 = p.t2;
+        v2 = p.v2;
+        fire = b.fire;
+        bad = b.bad;
+        viol = b.viol;
+        reduce_partA(tp, t2, v2);
+        reduce_partB(fire, bad, viol);
+        if (lane == 0) a.decide_cnt[r] = 0u;
+    } else if (wv != 0) {
+        return;
+    }
+    RealCtl c = c0;
+    if (!commit_step(a, ri, fire, bad, viol, c, lane == 0u)) {
+        if (lane == 0u) a.ctl[r] = c;
+        return;
+    }
+    finalize_A<MS, MR>(a, net_view(a), ri, r, tp, t2, v2, c);
+}
+
 // Explicit instantiations, smallest fitting one is used: Schlogl-like
 // (1 species, <= 4 reactions), <= 3 x 4, <= 5 x 8 (Brusselator), <= 8 x 16
 // and the general case.
@@ -1581,6 +1706,7 @@ __global__ __launch_bounds__(1024) void commit_kernel(KernelArgs a) {
     template __global__ void phase_a_kernel<MS, MR, false>(KernelArgs);     \
     template __global__ void phase_a_kernel<MS, MR, true>(KernelArgs);      \
     template __global__ void decide_kernel<MS, MR>(KernelArgs);             \
+    template __global__ void commit_decide_kernel<MS, MR>(KernelArgs);      \
     template __global__ void phase_b_kernel<MS, MR, false, false>(KernelArgs); \
     template __global__ void phase_b_kernel<MS, MR, false, true>(KernelArgs);
 #define PLIS_INST_EP(MS, MR)                                                \

@@ -110,6 +110,7 @@ struct KernelArgs {
     const int32_t* comp;
     PartA* partA_tile;
     PartA* partA_chunk;      // decide_kernel: one partial per (realization, chunk of its tiles)
+    PartB* partB_chunk;      // commit_decide_kernel: the same for the leap pass's sums
     uint32_t* decide_cnt;    // decide_kernel: chunks done per realization (zero between steps)
     uint32_t decide_chunks;  // decide_kernel: blocks per realization (1 .. kDecideChunks)
     PartB* partB_tile;
@@ -143,6 +144,9 @@ struct KernelArgs {
     int32_t judge_on;
     int32_t props_in_finish; // the leap pass proposes the next step's tau
     int32_t mlp_act;         // activation of the network: 0 ReLU, 1 GELU
+    // the tau pass of an iteration in the folded order: it runs on the counts
+    // the leap pass just wrote, for the step after this one
+    uint32_t tau_spec;
     const float* mlp_w32;    // fp32 master blob
     const void* mlp_q;       // packed bf16 copies
     const float* mlp_norm;   // feature / target standardisation
