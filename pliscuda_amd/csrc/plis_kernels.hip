@@ -708,13 +708,13 @@ __global__ __launch_bounds__(decide_threads<MS>()) void decide_kernel(KernelArgs
     }
     if (threadIdx.x == 0) {
         a.partA_chunk[r * nch + chunk] = PartA{tp, t2, v2, 0u};
-        __threadfence();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         s_last = atomicAdd(&a.decide_cnt[r], 1u) == nch - 1u ? 1u : 0u;
     }
     __syncthreads();
     if (!s_last || threadIdx.x >= 64u) return;
     // the last block of the realization: its first wavefront decides
-    __threadfence();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     const uint32_t lane = threadIdx.x;
     PartA p{kInf, kInf, 0xFFFFFFFFu, 0u};
     if (lane < nch) p = a.partA_chunk[r * nch + lane];
@@ -1666,13 +1666,13 @@ __global__ __launch_bounds__(decide_threads<MS>()) void commit_decide_kernel(Ker
         if (threadIdx.x == 0) {
             a.partA_chunk[r * nch + chunk] = PartA{tp, t2, v2, 0u};
             a.partB_chunk[r * nch + chunk] = PartB{fire, bad, viol};
-            __threadfence();
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
             s_last = atomicAdd(&a.decide_cnt[r], 1u) == nch - 1u ? 1u : 0u;
         }
         __syncthreads();
         if (!s_last || wv != 0) return;
         // the last block of the realization: its first wavefront finishes
-        __threadfence();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         PartA p{kInf, kInf, 0xFFFFFFFFu, 0u};
         PartB b{0ull, 0u, 0u};
         if (lane < nch) {
