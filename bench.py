@@ -6,7 +6,10 @@
 A "step" is one iteration of the device-resident step loop over the whole
 workload: one tau pass (propensities, Cao tau', tau'', global min) and one
 leap pass (Poisson firings, gather-form diffusion, critical event, negativity
-check, commit or leap-back).  The headline workload is the 1,048,576-
+check, commit or leap-back).  With plain tau the iteration runs the leap pass,
+then the tau pass on the counts it wrote, then one kernel that commits the
+step and decides the next (commit_decide_kernel): its time is reported as
+commit_ms_avg and decide_ms_avg is the empty slot between them.  The headline workload is the 1,048,576-
 subvolume Brusselator random mesh (128x128x64, degree 4..12, weighted edges,
 5 species, 8 reactions) with plain Cao tau; inputs are resident in HBM
 before timing starts.  value = accepted subvolume-leaps per second
@@ -431,7 +434,8 @@ def main():
         "leapbacks": res["leapbacks"],
         "crit_events": res["crit_events"],  # steps that ended at tau'' (a critical event fired)
         "subvolume_simtime_per_s": res["subvolume_simtime"] / res["seconds"],
-        "kernels": {"tau_ms_avg": res["tau_ms"], "leap_ms_avg": res["leap_ms"],
+        "kernels": {"step_order": "leap pass, tau pass on its counts, commit_decide_kernel",
+                    "tau_ms_avg": res["tau_ms"], "leap_ms_avg": res["leap_ms"],
                     "decide_ms_avg": res["decide_ms"], "commit_ms_avg": res["commit_ms"],
                     "iteration_ms_events": res["iter_ms_events"],
                     "tau_alg_bytes": bytes_tau, "leap_alg_bytes": bytes_leap,
