@@ -120,12 +120,18 @@ int plis_run_until(plis_ctx *ctx, double t_end, double t_pause, uint64_t step_li
 int plis_arm(plis_ctx *ctx, double t_end, double t_pause, uint64_t step_limit);
 
 /* Run exactly n step iterations (one tau pass + one leap pass per
- * iteration, whatever the realizations' clocks) -- the benchmark's unit. */
+ * iteration, whatever the realizations' clocks) -- the benchmark's unit.
+ * With plain tau an iteration runs the leap pass, the tau pass on the counts
+ * it wrote and one kernel that commits the step and decides the next; a
+ * call (this one, plis_run_until, plis_profile*) first decides the step on
+ * the current counts, which gives the values already in place, if any. */
 int plis_iterate(plis_ctx *ctx, uint32_t n);
 
 /* Time n iterations with HIP events recorded on the context's stream around
  * every kernel launch; out[0] = total ms, then the summed ms of the tau pass,
- * the decide kernel, the leap pass and the commit kernel (out[1..4]). */
+ * the decide kernel, the leap pass and the commit kernel (out[1..4]).  With
+ * plain tau, out[4] times the kernel that commits and decides and out[2]
+ * is 0 (the call's first decision is made before the timed iterations). */
 int plis_profile(plis_ctx *ctx, uint32_t n, double *out);
 
 /* As plis_profile, with the leap pass split by kernel: out[0..4] as there,
