@@ -186,6 +186,38 @@ def test_batch_equals_single():
     ctx1.close()
 
 
+def _iterate_in_calls(net, topo, x0, calls, eps, nc, seed):
+    ctx = capi.Context(net, [topo], [x0], eps=eps, nc=nc, seed=seed)
+    ctx.arm(t_end=float("inf"))
+    for n in calls:
+        ctx.iterate(n)
+    x, st = ctx.get_state(0), ctx.stats(0)
+    ctx.close()
+    return x, st
+
+
+@pytest.mark.parametrize("case", ["mesh", "leapbacks"])
+def test_iterations_split_into_calls_equal_one_call(case):
+    """Every call that runs iterations starts by deciding the step on the
+    current counts; a decision already in place (or a leap-back's halved
+    tau) must come out the same, so splitting the iterations into calls
+    changes nothing -- including calls that end right after a leap-back."""
+    if case == "mesh":
+        net, topo, x0 = H.small_bruss()
+        eps, nc, seed, calls = 0.03, 10, 5, (1, 2, 3, 5, 9)
+    else:  # the leap-back case of test_traj_leapbacks
+        net, topo, x0 = H.low_count_case(seed=23)
+        eps, nc, seed, calls = 0.9, 2, 1, (1,) * 10 + (7, 13, 29, 61, 80)
+    n = sum(calls)
+    xa, sa = _iterate_in_calls(net, topo, x0, (n,), eps, nc, seed)
+    xb, sb = _iterate_in_calls(net, topo, x0, calls, eps, nc, seed)
+    assert np.array_equal(xa, xb)
+    for f in ("steps", "leapbacks", "firings", "crit_events", "t"):
+        assert sa[f] == sb[f], (f, sa[f], sb[f])
+    if case == "leapbacks":
+        assert sa["leapbacks"] >= 1
+
+
 def test_launch_shape_does_not_change_results():
     wl = W.schlogl(n=16)
     outs = []
