@@ -951,7 +951,13 @@ __device__ __forceinline__ bool leap_subvolume(const KernelArgs& a, const NetVie
 constexpr uint32_t kQueue = 256;               // queued draws per wavefront
 constexpr uint32_t kQueueReact = 0x80000000u;  // reaction code: flag | lane << 8 | j
 constexpr uint32_t kQueueDefer = kQueue + 1u;  // count bump that sends the tile to the redo pass
-constexpr float kPtrsSureF = 10.0001f;         // a float mean above it is a double mean >= 10
+// A hop's float mean is within 8 * 2^-24 (relative) of its double mean, so a
+// float mean at or below kPtrsMaybeF is a double mean below 10 (at most
+// 9.99990 * (1 + 5e-7)): the walk defers a tile with a float mean above it,
+// and every queued draw of a tile it does not defer has a double mean below
+// 10 (a deferred tile whose means are below 10 after all gets the same
+// values from the redo pass's complete sampler).
+constexpr float kPtrsMaybeF = 9.9999f;
 constexpr uint32_t kMaxTileEdges = 1u << 27;   // hop codes hold the entry offset in 27 bits
 
 // Queue append by the whole wavefront (every lane calls it at the same
@@ -1089,7 +1095,7 @@ __device__ __forceinline__ uint32_t walk_fwd_draws(const KernelArgs& a, const Ne
         for (int w = 0; w < 2 * MS; ++w) {
             gw[w] = coarse_bound(lw[w]);
             any = any || lw[w] > 0.0f;
-            if constexpr (!REDO) defer = defer || lw[w] > kPtrsSureF;
+            if constexpr (!REDO) defer = defer || lw[w] > kPtrsMaybeF;
         }
         if (__ballot(any) == 0ull) continue;
         const U4 o = philox10(vo, u, sv.step, c3hop, k0, k1);  // the coarse block (MS <= 8)
@@ -1285,18 +1291,10 @@ __device__ __forceinline__ bool leap_tile_fwd(const KernelArgs& a, const NetView
     bool complete = true;
     if constexpr (!REDO) {
         const uint32_t cnt = wave_uniform(qn);
+        // (a deferred tile's count is above kQueue: a draw whose mean may be
+        // 10 or more needs PTRS, so nothing queued is drawn and the redo
+        // pass draws all of it)
         complete = cnt <= kQueue;
-        if (complete) {
-            // a queued hop whose double mean is >= 10 needs PTRS: nothing
-            // queued is drawn then, the redo pass draws all of it
-            bool big = false;
-            for (uint32_t q = lane; q < cnt; q += (uint32_t)kTile) {
-                const uint32_t code = s_q[q].x;
-                if (!(code & kQueueReact))
-                    big = big || !(queued_hop<MS>(a, n, ri, X, v0, sv, s_x, eb, s_c, code).lam < kPoissonInvMax);
-            }
-            complete = __ballot(big) == 0ull;
-        }
         if (complete) {
             for (uint32_t q = lane; q < cnt; q += (uint32_t)kTile)
                 drain_fwd_draw<MS>(a, n, ri, X, v0, sv, s_x, s_del, eb, s_c, s_q[q], fire);

@@ -64,11 +64,16 @@ def _coarse_byte(o, w):
     return (o[(w >> 2) & 3] >> (8 * (w & 3))) & 0xFF
 
 
+def _hop_mean_f(tau, d, x):
+    """A hop's mean in single precision like the kernel: tau_f ((d_f x_f) w), w = 1."""
+    return F32(tau) * (F32(F32(d) * F32(x)) * F32(1.0))
+
+
 def _hop_coarse_zero_f(tau, d, x, hb):
     """The walk's zero shortcut for a hop, on the top byte of its word and in
     single precision like the kernel: mean l = tau_f ((d_f x_f) w) with
     w = 1, settled iff hb <= (255 - 2^-12) - l (256 + 2^-10)."""
-    lf = F32(tau) * (F32(F32(d) * F32(x)) * F32(1.0))
+    lf = _hop_mean_f(tau, d, x)
     if not lf > 0:
         return True
     g = F32(F32(255.0 - 2.0 ** -12) - F32(lf * F32(256.0 + 2.0 ** -10)))
@@ -83,7 +88,8 @@ def _react_coarse_zero(lam, hb):
 
 def _predict_step0(net, topo, x0, eps, seed=1, nc=10):
     """Per tile of the first leap pass: number of draws the zero shortcut
-    leaves open (queued) and whether a mean >= 10 draw occurs.  Unweighted
+    leaves open (queued) and whether a draw with a mean that may be 10 or
+    more occurs (hops: float mean above 9.9999; reactions: double mean >= 10).  Unweighted
     symmetric topology, one compartment, at most 8 species, realization 0,
     step 0, attempt 0."""
     S, R, N = net["S"], net["R"], topo["N"]
@@ -113,7 +119,8 @@ def _predict_step0(net, topo, x0, eps, seed=1, nc=10):
                     if d[s] > 0 and xs >= nc and xs > 0:
                         if not _hop_coarse_zero_f(tau, d[s], xs, _coarse_byte(o, 2 * s + dirbit)):
                             und[t] += 1
-                            big[t] |= tau * ((d[s] * float(xs)) * 1.0) >= 10.0
+                            # the walk defers a tile with a float mean above 9.9999
+                            big[t] |= bool(_hop_mean_f(tau, d[s], xs) > F32(9.9999))
         o, oblk = None, -1
         for j in range(R):
             aj = k[j]
