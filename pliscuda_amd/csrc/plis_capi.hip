@@ -362,8 +362,10 @@ static KernelArgs make_args(plis_ctx* c) {
 static const int kVariantS[] = {1, 3, 5, 8, 64};
 static const int kVariantR[] = {4, 4, 8, 16, 128};
 
-// Kernels of one step: 0 tau pass (learned and/or Cao), 1 decide, 2 leap
-// pass, 3 commit; 5 = training records of one realization.  In a learned
+// Kernel groups: 0 tau pass (learned and/or Cao), 1 decide, 2 leap pass,
+// 3 commit; 4 tau pass on the counts the leap pass wrote and 6 commit +
+// decide (the folded order of plain tau); 5 = training records of one
+// realization.  In a learned
 // context the end of the edge-parallel leap pass also proposes the next
 // step's tau (learned_finish_kernel); learned_tau_kernel covers the steps
 // whose proposals it could not make.
