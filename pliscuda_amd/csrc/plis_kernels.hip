@@ -426,7 +426,11 @@ __device__ __forceinline__ void tau_in_edge(int S, const double (&du)[MS], const
     const uint32_t nc1 = nc > 1 ? (uint32_t)nc : 1u;
 #pragma unroll
     for (int s = 0; s < MS; ++s) {
-        if (s >= S) break;
+        // the small instantiations sum every species of the bound (the sums
+        // of species S.. are never read; xu and du hold valid padding values
+        // for them), so the loop has no runtime exit per species
+        if constexpr (MS > 5)
+            if (s >= S) break;
         uint32_t xe = xu[s] >= nc1 ? xu[s] : 0u;
         asm("" : "+v"(xe));  // select the count, not its conversion (one select instead of two)
         const double ain = (du[s] * (double)xe) * wq;
