@@ -927,8 +927,8 @@ __device__ __forceinline__ bool leap_subvolume(const KernelArgs& a, const NetVie
 // Each undirected edge {v, u}, v < u, is drawn once, by the tile of its
 // lower end: the tile's forward-CSR entries are contiguous, lane l takes
 // entries l, l+64, ... (coalesced loads, no divergence from uneven degrees),
-// reads the owning subvolume's index in the tile from the per-entry owner
-// bytes (fown), and draws both hop directions of every species from the
+// each one 8-byte word that holds the neighbour, the owning subvolume's
+// index in the tile and the weight, and draws both hop directions of every species from the
 // edge's coarse Philox block; then lane i draws the reactions of subvolume
 // v0+i (one coarse block per 16 reactions).
 // Balances of the tile's own subvolumes collect in LDS, those of other
@@ -944,7 +944,8 @@ __device__ __forceinline__ bool leap_subvolume(const KernelArgs& a, const NetVie
 // the fine bits, the float squeeze, then the double search where the
 // squeeze cannot decide -- so the fine block and the search run once per 64
 // open draws instead of once per draw site.  A tile whose queue
-// overflows, or that holds a draw of mean >= 10 (PTRS), applies nothing
+// overflows, or that holds a draw whose mean may be 10 or more (PTRS; see
+// kPtrsMaybeF), applies nothing
 // from its queue and goes on the redo list: phase_b_redo_kernel walks it
 // again and draws every draw the zero shortcut leaves open (the same test
 // on the same inputs) with the complete sampler.  Draws are functions of
