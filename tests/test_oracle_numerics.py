@@ -261,3 +261,32 @@ def test_coarse_bound_settles_only_zero_draws():
         assert ffi.poisson_inversion(float(lam_max[a]), word) == 0
     # a mean above 1 is never settled
     assert not settled[lf > 1.0].any()
+
+
+def test_float_mean_ptrs_threshold_is_conservative():
+    """The leap walk sends a tile to the redo pass when a hop's float mean
+    tau_f ((d_f x_f) w) exceeds 9.9999f; a tile it keeps is drawn with the
+    inversion, which needs every double mean tau ((d x) w) below 10.  Check
+    that a float mean at or below the threshold never has a double mean of
+    10 or more, on means packed around 10 (random d, w, counts and the tau
+    that puts the double mean within 2e-4 of 10), including the largest
+    counts."""
+    F = np.float32
+    thr = F(9.9999)
+    rng = np.random.default_rng(7)
+    n = 400_000
+    d = np.exp(rng.uniform(np.log(1e-3), np.log(1e3), n))
+    w = rng.uniform(0.5, 1.5, n).astype(F)  # weights are floats
+    x = rng.integers(1, 2**32, n, dtype=np.uint64)
+    x[: n // 4] = rng.integers(1, 1000, n // 4, dtype=np.uint64)
+    xd = x.astype(np.float64)
+    target = 10.0 * (1.0 + rng.uniform(-2e-4, 2e-4, n))
+    tau = target / ((d * xd) * w.astype(np.float64))
+    lam = tau * ((d * xd) * w.astype(np.float64))
+    lf = tau.astype(F) * ((d.astype(F) * xd.astype(F)) * w)
+    kept = lf <= thr
+    assert kept.sum() > n // 10 and (~kept).sum() > n // 10  # both sides of the threshold are exercised
+    assert not np.any(lam[kept] >= 10.0), "a kept tile would draw a mean >= 10 with the inversion"
+    # the float mean's relative error stays inside the 8 * 2^-24 the threshold assumes
+    rel = np.abs(lf.astype(np.float64) - lam) / lam
+    assert rel.max() < 8 * 2.0 ** -24
