@@ -633,10 +633,13 @@ void phase_a_kernel(KernelArgs a) {
 // One block per realization: min tau' / argmin tau'' over the realization's
 // tile partials (written by phase A; the kernel boundary makes them visible),
 // then the step decision.
-// Threads of decide_kernel: wide blocks read the tile partials in fewer
-// rounds; the large variants stay at 256 to keep their registers.
+// Threads of decide_kernel and commit_decide_kernel: 4 wavefronts.  With
+// 1,024 tile partials per chunk block that is 4 loads per thread, one
+// round; 16-wavefront blocks, one load per thread, measured slower (the
+// barrier and the 16-way reduction outweigh the round they save:
+// commit_decide_kernel 12.8 -> 9.8 us on the Brusselator mesh).
 template <int MS>
-constexpr uint32_t decide_threads() { return MS <= 5 ? 1024u : 256u; }
+constexpr uint32_t decide_threads() { return 256u; }
 
 // decide_kernel runs decide_chunks blocks per realization (one per 1,024
 // tile partials of the largest realization, at most kDecideChunks): each
